@@ -1,0 +1,311 @@
+#!/usr/bin/env python3
+"""Benchmark driver: one JSON line with the headline measurement.
+
+Headline (``value``): fused horizontal diffusion on a 512 x 512 x 160
+float64 domain, halo 3, 64-byte aligned, layout (2, 1, 0), fields U[0,1)
+from seed 0 - the plain-FP64 configuration the metric is quoted on.  A step
+is one hdiff launch over the whole domain with the fields resident in HBM;
+the caches (L2 and the 256 MB Infinity Cache) are flushed before every
+timed launch, and each launch is timed with HIP events on the stream it
+runs on.  ``value`` = compulsory bytes per launch / mean launch time.
+
+Alongside, in the same process: GPU STREAM copy/scale/add/triad on three
+2^29-element float64 arrays (the measured bandwidth ceiling), the 5-point
+Laplacian on 512 x 512 x 160, vertical advection at 512 x 512 x 160 and
+256 x 256 x 80 (float64, and float32 storage with float64 arithmetic), and
+the OpenMP CPU baseline of hdiff on the same fields timed on this host.
+"""
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from stencil_benchmarks_amd import native  # noqa: E402
+from stencil_benchmarks_amd.device import DeviceBuffer, DeviceField, geometry  # noqa: E402
+from stencil_benchmarks_amd.tools import array  # noqa: E402
+
+METRIC = "hdiff/vadv HBM GB/s & % of peak; batched-serve p50/p99 + speedup vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(message):
+    print(f"[bench] {message}", file=sys.stderr, flush=True)
+
+
+def make_host_fields(domain, halo, dtype, count, seed, alignment=64):
+    shape = tuple(d + 2 * h for d, h in zip(domain, halo))
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        a = array.alloc_array(shape, dtype, (2, 1, 0), alignment, halo)
+        a[...] = rng.random(shape, dtype=np.float64).astype(dtype)
+        out.append(a)
+    return out
+
+
+def device_fields_random(domain, halo, dtype, count, seed, alignment=64):
+    """Device fields filled on the GPU (counter hash over the whole span)."""
+    shape = tuple(d + 2 * h for d, h in zip(domain, halo))
+    template = array.alloc_array(shape, dtype, (2, 1, 0), alignment, halo)
+    fields = []
+    code = native.SB_F64 if np.dtype(dtype) == np.float64 else native.SB_F32
+    for f in range(count):
+        dev = DeviceField(template, halo)
+        n = dev.span // template.itemsize
+        native.call("sb_stream_fill_random", code, ctypes.c_void_p(dev.base), n, seed * 16 + f)
+        fields.append(dev)
+    strides = [s // template.itemsize for s in template.strides]
+    return fields, geometry(domain, strides, halo)
+
+
+def time_launches(launch, steps, warmup, flush=True):
+    """Warm up, then time `steps` launches; returns per-launch seconds and wall."""
+    for _ in range(warmup):
+        launch(native.launch_options(), None)
+    native.call("sb_synchronize")
+    times = []
+    start = time.perf_counter()
+    for _ in range(steps):
+        seconds = ctypes.c_double(0.0)
+        launch(native.launch_options(flush=flush), ctypes.byref(seconds))
+        times.append(seconds.value)
+    native.call("sb_synchronize")
+    wall = time.perf_counter() - start
+    return np.array(times), wall
+
+
+def hdiff_bytes(domain, itemsize):
+    nx, ny, nz = domain
+    return ((nx + 4) * (ny + 4) * nz + 2 * nx * ny * nz) * itemsize
+
+
+def vadv_bytes(domain, itemsize):
+    nx, ny, nz = domain
+    return (5 * nx * ny * nz + (nx + 1) * ny * nz) * itemsize
+
+
+def lap_bytes(domain, itemsize):
+    nx, ny, nz = domain
+    return ((nx + 2) * (ny + 2) * nz + nx * ny * nz) * itemsize
+
+
+def bench_hdiff_headline(steps, warmup):
+    domain, halo = (512, 512, 160), (3, 3, 3)
+    host = make_host_fields(domain, halo, np.float64, 3, seed=0)
+    host[2][...] = 0
+    devs = [DeviceField(a, halo) for a in host]
+    for d, a in zip(devs, host):
+        d.upload(a)
+    geom = geometry(domain, [s // 8 for s in host[0].strides], halo)
+    ptrs = [ctypes.c_void_p(d.pointer()) for d in devs]
+
+    def launch(opts, seconds):
+        native.call("sb_hdiff", native.SB_F64, *ptrs, geom, opts, seconds)
+
+    times, wall = time_launches(launch, steps, warmup)
+    warm, _ = time_launches(launch, max(3, steps // 2), 0, flush=False)
+    result = devs[2]
+    gpu_out = np.ndarray(host[2].shape, np.float64,
+                         buffer=np.zeros(result.span, np.uint8), strides=host[2].strides)
+    result.download(gpu_out)
+    return dict(domain=domain, halo=halo, host=host, gpu_out=gpu_out, times=times, wall=wall,
+                warm_times=warm, bytes=hdiff_bytes(domain, 8))
+
+
+def bench_stream(steps):
+    n = 1 << 29
+    bufs = [DeviceBuffer(n * 8) for _ in range(3)]
+    for i, b in enumerate(bufs):
+        native.call("sb_stream_fill_random", native.SB_F64, ctypes.c_void_p(b.ptr), n, 100 + i)
+    out = {}
+    words = {"copy": 2, "scale": 2, "add": 3, "triad": 3}
+    for variant in (0, 3):
+        for op, code in native.STREAM_OPS.items():
+            opts = native.launch_options(variant=variant, block=(256, 1, 1), unroll=4)
+            best = None
+            for _ in range(max(3, steps)):
+                seconds = ctypes.c_double(0.0)
+                native.call("sb_stream_kernel", native.SB_F64, code,
+                            *[ctypes.c_void_p(b.ptr) for b in bufs], n, 3.0, opts,
+                            ctypes.byref(seconds))
+                best = seconds.value if best is None else min(best, seconds.value)
+            key = op if variant == 0 else f"{op}_nt"
+            out[key] = words[op] * n * 8 / best / 1e9
+    for b in bufs:
+        b.free()
+    return out
+
+
+def bench_lap(steps):
+    domain, halo = (512, 512, 160), (3, 3, 3)
+    devs, geom = device_fields_random(domain, halo, np.float64, 2, seed=3)
+    ptrs = [ctypes.c_void_p(d.pointer()) for d in devs]
+
+    def launch(opts, seconds):
+        native.call("sb_laplacian", native.SB_F64, 1, 1, 0, *ptrs, geom, opts, seconds)
+
+    times, _ = time_launches(launch, steps, 1)
+    return lap_bytes(domain, 8) / times.mean() / 1e9, times.mean()
+
+
+def bench_vadv(domain, dtype, precision, steps, variant=0):
+    halo = (3, 3, 3)
+    devs, geom = device_fields_random(domain, halo, dtype, 8, seed=4)
+    ptr_array = (ctypes.c_void_p * 8)(*[d.pointer() for d in devs])
+
+    def launch(opts, seconds):
+        opts.variant = variant
+        native.call("sb_vadv", precision, 0, ptr_array, geom, opts, seconds)
+
+    times, _ = time_launches(launch, steps, 1)
+    itemsize = np.dtype(dtype).itemsize
+    return vadv_bytes(domain, itemsize) / times.mean() / 1e9, times.mean()
+
+
+def bench_hdiff_f32(steps):
+    domain, halo = (512, 512, 160), (3, 3, 3)
+    devs, geom = device_fields_random(domain, halo, np.float32, 3, seed=5)
+    ptrs = [ctypes.c_void_p(d.pointer()) for d in devs]
+
+    def launch(opts, seconds):
+        native.call("sb_hdiff", native.SB_F32_F64, *ptrs, geom, opts, seconds)
+
+    times, _ = time_launches(launch, steps, 1)
+    return hdiff_bytes(domain, 4) / times.mean() / 1e9, times.mean()
+
+
+def cpu_baseline(headline, budget_s=12.0):
+    """OpenMP port of the reference CPU hdiff on the headline fields."""
+    from oracle import oracle
+
+    lib = oracle.cpu_lib()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    domain, halo = headline["domain"], headline["halo"]
+    inp, coeff, out = headline["host"]
+    strides = [s // 8 for s in inp.strides]
+
+    def origin(a):
+        return ctypes.c_void_p(a.ctypes.data + sum(s * h for s, h in zip(a.strides, halo)))
+
+    args = [origin(inp), origin(coeff), origin(out)] + [ctypes.c_ssize_t(v) for v in (*domain, *strides)]
+    lib.cpu_hdiff_f64(*args, ctypes.c_int(threads))  # warm-up (first touch, thread spin-up)
+    times = []
+    spent = 0.0
+    while spent < budget_s and len(times) < 20:
+        t = lib.cpu_hdiff_f64(*args, ctypes.c_int(threads))
+        times.append(t)
+        spent += t
+    inner = tuple(slice(h, h + d) for d, h in zip(domain, halo))
+    diff = float(np.abs(out[inner] - headline["gpu_out"][inner]).max())
+    best = float(np.median(times))
+    return {
+        "value": headline["bytes"] / best / 1e9,
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"hdiff 512x512x160 float64 full domain, OpenMP Classic 8x8x1 port, "
+                  f"{len(times)} runs ({spent:.1f} s), median run",
+        "seconds_per_pass": best,
+        "max_abs_diff_vs_gpu": diff,
+    }
+
+
+def main():
+    parser = argparse.ArgumentParser()
+    parser.add_argument("--gpus", type=int, default=1)
+    parser.add_argument("--steps", type=int, default=20)
+    parser.add_argument("--warmup", type=int, default=3)
+    parser.add_argument("--quick", action="store_true", help="headline only")
+    parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = parser.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the stencil path shards by subdomain, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    native.init(0)
+    info = native.device_info(0)
+    log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}")
+
+    log("hdiff 512x512x160 float64 (headline)")
+    head = bench_hdiff_headline(args.steps, args.warmup)
+    mean_s = float(head["times"].mean())
+    achieved = head["bytes"] / mean_s / 1e9
+    log(f"hdiff: {achieved:.1f} GB/s, mean launch {mean_s * 1e6:.1f} us")
+
+    extras = {}
+    if not args.quick:
+        log("STREAM 3 x 2^29 float64")
+        extras["stream_gbs"] = bench_stream(args.steps // 4 + 3)
+        log(f"STREAM {extras['stream_gbs']}")
+        log("laplacian 512x512x160 float64")
+        extras["laplacian_gbs"], _ = bench_lap(args.steps)
+        log("vadv")
+        extras["vadv_512x512x160_f64_gbs"], _ = bench_vadv((512, 512, 160), np.float64,
+                                                           native.SB_F64, args.steps)
+        extras["vadv_256x256x80_f64_gbs"], _ = bench_vadv((256, 256, 80), np.float64,
+                                                          native.SB_F64, args.steps)
+        extras["vadv_256x256x80_f32s_f64c_gbs"], _ = bench_vadv((256, 256, 80), np.float32,
+                                                                native.SB_F32_F64, args.steps)
+        extras["vadv_512x512x160_f32s_f64c_gbs"], _ = bench_vadv((512, 512, 160), np.float32,
+                                                                 native.SB_F32_F64, args.steps)
+        extras["hdiff_512x512x160_f32s_f64c_gbs"], _ = bench_hdiff_f32(args.steps)
+        extras["hdiff_512x512x160_f64_warm_gbs"] = head["bytes"] / float(head["warm_times"].mean()) / 1e9
+        triad = max(extras["stream_gbs"]["triad"], extras["stream_gbs"]["triad_nt"])
+        extras["hdiff_frac_of_triad"] = achieved / triad
+        extras["vadv_512x512x160_f64_frac_of_triad"] = extras["vadv_512x512x160_f64_gbs"] / triad
+        extras["vadv_256x256x80_f64_frac_of_triad"] = extras["vadv_256x256x80_f64_gbs"] / triad
+
+    baseline = None
+    if not args.no_cpu:
+        log("CPU baseline (OpenMP port) on the headline fields")
+        baseline = cpu_baseline(head)
+
+    record = {
+        "metric": METRIC,
+        "value": achieved,
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["wall"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: fields uniform[0,1) from seed 0, resident in HBM, caches flushed "
+                "before each timed launch",
+        "config": {
+            "workload": "hdiff 512x512x160 halo 3 float64, 64-byte aligned, layout (2,1,0)",
+            "domain": [512, 512, 160],
+            "halo": [3, 3, 3],
+            "kernel": "sb_hdiff variant 1 (LDS tile 64x16)",
+            "compulsory_bytes_per_launch": head["bytes"],
+            "device": info["name"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "cpu_baseline": baseline,
+        "kernel_mean_us": mean_s * 1e6,
+        "extras": extras,
+    }
+    print(json.dumps(record))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+/*
+ * sbmi355x.h - C ABI of libsbmi355x.so, the MI355X (gfx950) stencil and
+ * STREAM library.
+ *
+ * Conventions (shared by every entry point):
+ *  - Every function returns 0 on success and a non-zero status on failure;
+ *    sb_last_error() then returns a message (thread-local).  This replaces
+ *    the reference's "return 1 and print to stderr" launcher convention
+ *    (stencils/cuda_hip/templates/base.j2:48-55) whose stderr the Python side
+ *    captured by redirecting process-wide file descriptors
+ *    (tools/compilation.py:155-196); nothing here touches fds.
+ *  - Field pointers are DEVICE pointers to the first interior point (the
+ *    element at index == halo), exactly like the reference's
+ *    data_ptr(device_array, halo) arguments (stencils/cuda_hip/mixin.py:162-175).
+ *    The geometry that the reference baked into generated code as literals
+ *    (domain, strides) is passed at run time in sb_geom.
+ *  - Compute entry points launch on the library's stream of the current
+ *    device.  If `seconds` is non-NULL the call runs launch->dry_runs untimed
+ *    launches, optionally evicts the caches (launch->flush), brackets one
+ *    launch with HIP events on that stream, waits for it and stores the
+ *    elapsed time in seconds (the reference's "gpu" timer,
+ *    cuda_hip/templates/base.j2:145-160).  If `seconds` is NULL the launch is
+ *    asynchronous; call sb_synchronize().
+ *  - Precision codes: SB_F64 fields and arithmetic in float64; SB_F32 fields
+ *    and arithmetic in float32; SB_F32_F64 fields stored as float32, all
+ *    arithmetic in float64, results rounded once to float32.
+ *
+ * Paths cited below are relative to the reference's stencil_benchmarks/
+ * package directory.
+ */
+#ifndef SBMI355X_H
+#define SBMI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SB_ABI_VERSION 1
+
+enum sb_precision { SB_F64 = 0, SB_F32 = 1, SB_F32_F64 = 2 };
+
+enum sb_stream_op { SB_STREAM_COPY = 0, SB_STREAM_SCALE = 1, SB_STREAM_ADD = 2, SB_STREAM_TRIAD = 3 };
+
+/* Field geometry: interior extents, element strides of the logical axes
+ * (i, j, k) and the halo width that is addressable on each side.  Replaces
+ * the compile-time `domain` / `strides` template arguments
+ * (stencils/cuda_hip/mixin.py:113-123, stencils/base.py:115-121). */
+typedef struct sb_geom {
+    int64_t nx, ny, nz;
+    int64_t si, sj, sk;
+    int64_t hi, hj, hk;
+} sb_geom;
+
+/* Launch options; zero means "library default / autotuned".  Replaces the
+ * block_size / threads_per_block / unroll_factor / dry_runs parameters
+ * (stencils/cuda_hip/basic.py:43-48, horizontal_diffusion.py:40-47,
+ * vertical_advection.py:39-57, mixin.py:56). */
+typedef struct sb_launch {
+    int32_t variant;   /* kernel variant id, see the per-function notes */
+    int32_t bx, by, bz;
+    int32_t unroll;
+    int32_t dry_runs;
+    int32_t flush;     /* 1: evict L2 and Infinity Cache before the timed launch */
+    int32_t reserved;
+} sb_launch;
+
+typedef struct sb_device_info {
+    char name[256];
+    char arch[64];
+    int32_t compute_units;
+    int32_t wave_size;
+    int32_t max_threads_per_block;
+    int32_t xcds;
+    int64_t lds_per_block;
+    int64_t lds_per_cu;
+    int64_t total_mem;
+    int64_t l2_bytes;
+    int32_t clock_khz;
+    int32_t mem_clock_khz;
+    int32_t mem_bus_width;
+    int32_t reserved;
+} sb_device_info;
+
+/* ---- errors, lifecycle, device ------------------------------------------ */
+const char *sb_last_error(void);
+int sb_abi_version(void);
+/* Select and initialise a device (stream, events).  Replaces the implicit
+ * device setup of the generated launcher (cuda_hip/templates/base.j2:68-91). */
+int sb_init(int device);
+int sb_device_count(int *count);
+/* Device properties; replaces hipGetDeviceProperties in base.j2:76-91. */
+int sb_get_device_info(int device, sb_device_info *info);
+int sb_mem_info(size_t *free_bytes, size_t *total_bytes);
+
+/* ---- memory ---------------------------------------------------------------
+ * Replace api.Runtime.malloc / memcpy / device_synchronize
+ * (stencils/cuda_hip/api.py:54-95). */
+int sb_malloc(void **ptr, size_t bytes);
+int sb_free(void *ptr);
+int sb_host_malloc(void **ptr, size_t bytes); /* pinned host memory */
+int sb_host_free(void *ptr);
+int sb_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int sb_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int sb_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int sb_memset(void *dst, int value, size_t bytes);
+int sb_synchronize(void);
+/* Stream a scratch buffer larger than L2 + Infinity Cache through the
+ * device; the cold-cache counterpart of cycling `data_sets`
+ * (stencils/base.py:47-51, :152). */
+int sb_flush_caches(void);
+
+/* ---- basic stencils ------------------------------------------------------
+ * Computations of the generated basic kernels
+ * (stencils/cuda_hip/basic.py:101-132, templates/basic_1d.j2, basic_3d.j2),
+ * interior points only, with the arithmetic order of the verification
+ * oracles (stencils/base.py:180-254). */
+int sb_copy(int prec, const void *inp, void *out, const sb_geom *g,
+            const sb_launch *l, double *seconds);
+/* out = (inp[x + e_axis] + inp[x]) / 2 (base.py:191-205). */
+int sb_onesided_average(int prec, int axis, const void *inp, void *out,
+                        const sb_geom *g, const sb_launch *l, double *seconds);
+/* out = (inp[x + e_axis] + inp[x - e_axis]) / 2 (base.py:208-222). */
+int sb_symmetric_average(int prec, int axis, const void *inp, void *out,
+                         const sb_geom *g, const sb_launch *l, double *seconds);
+/* out = sum over enabled axes of ((2 inp - inp[+e]) - inp[-e]) (base.py:225-254). */
+int sb_laplacian(int prec, int along_x, int along_y, int along_z,
+                 const void *inp, void *out, const sb_geom *g,
+                 const sb_launch *l, double *seconds);
+
+/* ---- horizontal diffusion ------------------------------------------------
+ * Replaces kernel(double*, inp, coeff, out) of the hdiff variants
+ * (stencils/cuda_hip/horizontal_diffusion.py:50-128 and
+ * templates/horizontal_diffusion_*.j2); semantics of base.py:276-311.
+ * Requires hi, hj >= 2.  Variants: 0 default, 1 LDS-staged tile,
+ * 2 register j-scan with cross-lane exchange. */
+int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
+             const sb_geom *g, const sb_launch *l, double *seconds);
+
+/* ---- vertical advection --------------------------------------------------
+ * Replaces kernel(double*, ustage, upos, utens, utensstage, wcon, ccol,
+ * dcol, datacol) of the vadv variants (stencils/cuda_hip/vertical_advection.py:60-73,
+ * templates/vertical_advection_*.j2); semantics of base.py:349-501.
+ * `fields` holds the device pointers in the reference's argument order
+ * (base.py:329-337): 8 entries, or 16 when all_components != 0.  Only the
+ * *tensstage fields are written; ccol/dcol are used as scratch by
+ * variant 2 only; datacol is never touched.  Requires hi >= 1 (all
+ * components: hj >= 1 too).  Variants: 0 default, 1 Thomas coefficients
+ * held in LDS, 2 coefficients in the ccol/dcol fields. */
+int sb_vadv(int prec, int all_components, void *const *fields,
+            const sb_geom *g, const sb_launch *l, double *seconds);
+
+/* ---- STREAM ---------------------------------------------------------------
+ * Replaces the generated STREAM program (benchmarks_collection/stream/cuda_hip.j2:132-288).
+ * prec is SB_F64 or SB_F32.  sb_stream_kernel runs one of copy (c = a),
+ * scale (b = s c), add (c = a + b), triad (a = b + s c) over n elements.
+ * launch->bx = threads per block, ->unroll = vectors per thread,
+ * ->variant bit 0 = non-temporal loads, bit 1 = non-temporal stores. */
+int sb_stream_fill(int prec, void *x, size_t n, double value);
+/* x[i] = uniform [0,1) from a counter-based hash of (seed, i). */
+int sb_stream_fill_random(int prec, void *x, size_t n, uint64_t seed);
+int sb_stream_kernel(int prec, int op, void *a, void *b, void *c, size_t n,
+                     double scalar, const sb_launch *l, double *seconds);
+/* max over i of |x[i] - expected| / |expected| (closed-form check of
+ * stream/cuda_hip.j2:290-345). */
+int sb_stream_check(int prec, const void *x, size_t n, double expected,
+                    double *max_rel_err);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBMI355X_H */

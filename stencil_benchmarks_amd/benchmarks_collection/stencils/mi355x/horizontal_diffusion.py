@@ -1,0 +1,52 @@
+"""Horizontal diffusion on the MI355X.
+
+``Fused`` is the LDS-tiled fused kernel (csrc/hdiff.hip, variant 1): the
+input tile and its 2-point halo are staged in LDS and the Laplacian, flux
+limiter and update never round-trip HBM.  ``OnTheFly`` recomputes every
+neighbour Laplacian from cached loads, like the reference's OnTheFly
+schedule (stencils/cuda_hip/horizontal_diffusion.py:50-96).  Both use the
+oracle's operation order (stencils/base.py:284-307), bit-exact in FP64.
+"""
+
+import numpy as np
+
+from .... import native
+from ....benchmark import Parameter, ParameterError
+from ..base import HorizontalDiffusionStencil
+from .mixin import StencilMixin
+
+
+class HorizontalDiffusionMixin(StencilMixin):
+    block_size = Parameter("tile size (i fixed to 64; j in 8, 16, 32)", (64, 16, 1))
+
+    VARIANT = 1
+
+    def setup(self):
+        super().setup()
+        if self.VARIANT == 1 and (self.block_size[0] != 64 or self.block_size[1] not in (8, 16, 32)):
+            raise ParameterError(f"unsupported tile size {self.block_size}")
+
+    @property
+    def written_fields(self):
+        return ("out",)
+
+    def launch_options(self):
+        options = super().launch_options()
+        options.variant = self.variant or self.VARIANT
+        options.bx, options.by = self.block_size[0], self.block_size[1]
+        return options
+
+    def launch(self, pointers, options, seconds):
+        native.call("sb_hdiff", self.precision, *pointers, self.geom, options, seconds)
+
+    def compulsory_bytes(self):
+        nx, ny, nz = self.domain
+        return ((nx + 4) * (ny + 4) * nz + 2 * nx * ny * nz) * self.dtype_size
+
+
+class Fused(HorizontalDiffusionMixin, HorizontalDiffusionStencil):
+    VARIANT = 1
+
+
+class OnTheFly(HorizontalDiffusionMixin, HorizontalDiffusionStencil):
+    VARIANT = 3

@@ -1,0 +1,58 @@
+"""Vertical advection on the MI355X.
+
+One lane per (i, j) column with contiguous i, forward sweep and fused
+backward substitution in one kernel (csrc/vadv.hip).  ``Fused`` keeps the
+Thomas coefficients in LDS when nz fits (nz <= 160 in FP64 arithmetic) and
+in a global FP64 scratch otherwise; ``LdsColumn`` and ``GlobalScratch`` pin
+one storage.  Arithmetic follows stencils/base.py:415-473 (bit-exact in
+FP64; with float32 storage the default arithmetic is FP64 throughout).
+"""
+
+import ctypes
+
+import numpy as np
+
+from .... import native
+from ..base import VerticalAdvectionStencil
+from .mixin import StencilMixin
+
+
+class VerticalAdvectionMixin(StencilMixin):
+    VARIANT = 0
+    written_fields_are_read = True
+
+    @property
+    def written_fields(self):
+        return tuple(f"{c}tensstage" for c, _ in self.components)
+
+    def launch_options(self):
+        options = super().launch_options()
+        options.variant = self.variant or self.VARIANT
+        return options
+
+    def launch(self, pointers, options, seconds):
+        array = (ctypes.c_void_p * len(pointers))(*[p.value for p in pointers])
+        native.call(
+            "sb_vadv", self.precision, int(self.all_components), array, self.geom, options, seconds
+        )
+
+    def compulsory_bytes(self):
+        nx, ny, nz = self.domain
+        n = nx * ny * nz
+        ncomp = len(self.components)
+        # per component: stage, pos, tens, tensstage read + tensstage write;
+        # wcon (with its i+1 / j+1 neighbour column) read once
+        wcon = (nx + 1) * (ny + (1 if self.all_components else 0)) * nz
+        return (5 * ncomp * n + wcon) * self.dtype_size
+
+
+class Fused(VerticalAdvectionMixin, VerticalAdvectionStencil):
+    VARIANT = 0
+
+
+class LdsColumn(VerticalAdvectionMixin, VerticalAdvectionStencil):
+    VARIANT = 1
+
+
+class GlobalScratch(VerticalAdvectionMixin, VerticalAdvectionStencil):
+    VARIANT = 2

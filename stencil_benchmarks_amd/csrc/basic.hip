@@ -1,0 +1,285 @@
+// Basic stencils over the interior: copy, one-sided and symmetric average,
+// Laplacian.
+//
+// The host permutes the three logical axes so that kernel axis 0 is the one
+// with unit stride.  Each thread computes V consecutive points along it:
+// the centre and the off-axis neighbours are V-wide aligned vector loads
+// (one dwordx4 for V*sizeof(T) == 16), the two along-axis neighbours that
+// fall outside the vector are scalar loads, the rest are register shuffles.
+// A workgroup sweeps ROWS rows (points with equal axis-1/axis-2 index) so
+// every thread keeps several rows' loads in flight.
+//
+// Arithmetic follows the verification oracles of stencils/base.py:
+//   onesided  (inp[x+e] + inp[x]) / 2                       (base.py:204)
+//   symmetric (inp[x+e] + inp[x-e]) / 2                     (base.py:221)
+//   laplacian r = 0; r += (2 inp - inp[x+e]) - inp[x-e] per enabled axis in
+//             x, y, z order                                 (base.py:243-253)
+#include "sb_internal.h"
+
+namespace sb {
+namespace {
+
+enum { OP_COPY = 0, OP_AVG1 = 1, OP_AVG2 = 2, OP_LAP = 3 };
+
+struct BasicArgs {
+    int64_t n[3];       // extents of the permuted axes
+    int64_t s[3];       // element strides of the permuted axes (s[0] == 1 if V > 1)
+    int32_t axis;       // permuted averaging axis
+    int32_t lap_mask;   // bit o: Laplacian along ORIGINAL axis o
+    int32_t perm[3];    // permuted index of original axis o
+    int32_t pad;
+};
+
+constexpr int ROWS = 4;
+constexpr int BLOCK = 256;
+
+template <typename T, int V>
+struct alignas(sizeof(T) * V) Vec {
+    T e[V];
+};
+
+template <typename T, typename C, int OP, int V>
+__device__ __forceinline__ C scalar_point(const T *__restrict__ inp, int64_t p,
+                                          const BasicArgs &a) {
+    if constexpr (OP == OP_COPY) {
+        return C(inp[p]);
+    } else if constexpr (OP == OP_AVG1) {
+        const int64_t s = a.s[a.axis];
+        return (C(inp[p + s]) + C(inp[p])) / C(2);
+    } else if constexpr (OP == OP_AVG2) {
+        const int64_t s = a.s[a.axis];
+        return (C(inp[p + s]) + C(inp[p - s])) / C(2);
+    } else {
+        const C c = C(inp[p]);
+        C r = C(0);
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            if (a.lap_mask & (1 << o)) {
+                const int64_t s = a.s[a.perm[o]];
+                r += (C(2) * c - C(inp[p + s])) - C(inp[p - s]);
+            }
+        }
+        return r;
+    }
+}
+
+template <typename T, typename C, int OP, int V>
+__device__ __forceinline__ void vector_points(const T *__restrict__ inp, T *__restrict__ out,
+                                              int64_t p, const BasicArgs &a) {
+    using VT = Vec<T, V>;
+    const VT cv = *reinterpret_cast<const VT *>(inp + p);
+    VT res;
+    if constexpr (OP == OP_COPY) {
+        res = cv;
+    } else if constexpr (OP == OP_AVG1 || OP == OP_AVG2) {
+        C plus[V], minus[V];
+        if (a.axis == 0) {
+            const C right = C(inp[p + V]);
+            C left = C(0);
+            if constexpr (OP == OP_AVG2) left = C(inp[p - 1]);
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                plus[e] = e + 1 < V ? C(cv.e[e + 1]) : right;
+                minus[e] = e > 0 ? C(cv.e[e - 1]) : left;
+            }
+        } else {
+            const int64_t s = a.s[a.axis];
+            const VT pv = *reinterpret_cast<const VT *>(inp + p + s);
+#pragma unroll
+            for (int e = 0; e < V; ++e) plus[e] = C(pv.e[e]);
+            if constexpr (OP == OP_AVG2) {
+                const VT mv = *reinterpret_cast<const VT *>(inp + p - s);
+#pragma unroll
+                for (int e = 0; e < V; ++e) minus[e] = C(mv.e[e]);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+            if constexpr (OP == OP_AVG1)
+                res.e[e] = T((plus[e] + C(cv.e[e])) / C(2));
+            else
+                res.e[e] = T((plus[e] + minus[e]) / C(2));
+        }
+    } else {
+        C r[V];
+#pragma unroll
+        for (int e = 0; e < V; ++e) r[e] = C(0);
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            if (!(a.lap_mask & (1 << o))) continue;
+            const int q = a.perm[o];
+            C plus[V], minus[V];
+            if (q == 0) {
+                const C right = C(inp[p + V]);
+                const C left = C(inp[p - 1]);
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    plus[e] = e + 1 < V ? C(cv.e[e + 1]) : right;
+                    minus[e] = e > 0 ? C(cv.e[e - 1]) : left;
+                }
+            } else {
+                const int64_t s = a.s[q];
+                const VT pv = *reinterpret_cast<const VT *>(inp + p + s);
+                const VT mv = *reinterpret_cast<const VT *>(inp + p - s);
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    plus[e] = C(pv.e[e]);
+                    minus[e] = C(mv.e[e]);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < V; ++e) r[e] += (C(2) * C(cv.e[e]) - plus[e]) - minus[e];
+        }
+#pragma unroll
+        for (int e = 0; e < V; ++e) res.e[e] = T(r[e]);
+    }
+    *reinterpret_cast<VT *>(out + p) = res;
+}
+
+template <typename T, typename C, int OP, int V>
+__global__ __launch_bounds__(BLOCK) void basic_kernel(const T *__restrict__ inp,
+                                                      T *__restrict__ out, BasicArgs a) {
+    const int64_t i = (int64_t(blockIdx.y) * BLOCK + threadIdx.x) * V;
+    if (i >= a.n[0]) return;
+    const int64_t rows = a.n[1] * a.n[2];
+    const bool full = (i + V <= a.n[0]);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int64_t row = int64_t(blockIdx.x) * ROWS + r;
+        if (row >= rows) break;
+        const int64_t j = row % a.n[1];
+        const int64_t k = row / a.n[1];
+        const int64_t p = i * a.s[0] + j * a.s[1] + k * a.s[2];
+        if (V > 1 && full) {
+            vector_points<T, C, OP, V>(inp, out, p, a);
+        } else {
+            const int64_t last = a.n[0] - i < V ? a.n[0] - i : V;
+            for (int64_t e = 0; e < last; ++e)
+                out[p + e * a.s[0]] = T(scalar_point<T, C, OP, V>(inp, p + e * a.s[0], a));
+        }
+    }
+}
+
+template <int P, int OP>
+int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis, int lap_mask,
+                      const sb_launch *l, double *seconds) {
+    using T = typename Prec<P>::T;
+    using C = typename Prec<P>::C;
+    const T *inp = static_cast<const T *>(inp_v);
+    T *out = static_cast<T *>(out_v);
+    SB_REQUIRE(inp && out, "NULL field pointer");
+
+    // Permute axes: unit (smallest |stride|) axis first, others by stride.
+    const int64_t n_orig[3] = {g->nx, g->ny, g->nz};
+    const int64_t s_orig[3] = {g->si, g->sj, g->sk};
+    int order[3] = {0, 1, 2};
+    for (int x = 0; x < 3; ++x)
+        for (int y = x + 1; y < 3; ++y) {
+            const int64_t sx = s_orig[order[x]] < 0 ? -s_orig[order[x]] : s_orig[order[x]];
+            const int64_t sy = s_orig[order[y]] < 0 ? -s_orig[order[y]] : s_orig[order[y]];
+            if (sy < sx) {
+                const int t = order[x];
+                order[x] = order[y];
+                order[y] = t;
+            }
+        }
+    BasicArgs a{};
+    for (int q = 0; q < 3; ++q) {
+        a.n[q] = n_orig[order[q]];
+        a.s[q] = s_orig[order[q]];
+        a.perm[order[q]] = q;
+    }
+    a.axis = a.perm[axis];
+    a.lap_mask = lap_mask;
+    if (a.n[0] == 0 || a.n[1] == 0 || a.n[2] == 0) {
+        if (seconds) *seconds = 0.0;
+        return 0;
+    }
+
+    sb_geom pg = *g;
+    pg.si = a.s[0];
+    pg.sj = a.s[1];
+    pg.sk = a.s[2];
+    const void *ptrs[2] = {inp, out};
+    const int V = vector_width(sizeof(T), ptrs, 2, &pg);
+
+    Context *ctx = context();
+    if (!ctx) return 1;
+    const int64_t rows = a.n[1] * a.n[2];
+    const dim3 grid(unsigned((rows + ROWS - 1) / ROWS),
+                    unsigned((a.n[0] + int64_t(BLOCK) * V - 1) / (int64_t(BLOCK) * V)));
+    return timed_launch(l, seconds, [&]() -> hipError_t {
+        switch (V) {
+            case 4:
+                hipLaunchKernelGGL((basic_kernel<T, C, OP, (sizeof(T) == 4 ? 4 : 1)>), grid,
+                                   dim3(BLOCK), 0, ctx->stream, inp, out, a);
+                break;
+            case 2:
+                hipLaunchKernelGGL((basic_kernel<T, C, OP, 2>), grid, dim3(BLOCK), 0,
+                                   ctx->stream, inp, out, a);
+                break;
+            default:
+                hipLaunchKernelGGL((basic_kernel<T, C, OP, 1>), grid, dim3(BLOCK), 0,
+                                   ctx->stream, inp, out, a);
+                break;
+        }
+        return hipGetLastError();
+    });
+}
+
+template <int P>
+int launch_copy(const void *inp, void *out, const sb_geom *g, const sb_launch *l,
+                double *seconds) {
+    return launch_basic_prec<P, OP_COPY>(inp, out, g, 0, 0, l, seconds);
+}
+template <int P>
+int launch_avg1(const void *inp, void *out, const sb_geom *g, int axis, const sb_launch *l,
+                double *seconds) {
+    return launch_basic_prec<P, OP_AVG1>(inp, out, g, axis, 0, l, seconds);
+}
+template <int P>
+int launch_avg2(const void *inp, void *out, const sb_geom *g, int axis, const sb_launch *l,
+                double *seconds) {
+    return launch_basic_prec<P, OP_AVG2>(inp, out, g, axis, 0, l, seconds);
+}
+template <int P>
+int launch_lap(const void *inp, void *out, const sb_geom *g, int mask, const sb_launch *l,
+               double *seconds) {
+    return launch_basic_prec<P, OP_LAP>(inp, out, g, 0, mask, l, seconds);
+}
+
+}  // namespace
+}  // namespace sb
+
+using namespace sb;
+
+extern "C" {
+
+int sb_copy(int prec, const void *inp, void *out, const sb_geom *g, const sb_launch *l,
+            double *seconds) {
+    if (check_geom(g, 0, 0, 0)) return 2;
+    return SB_DISPATCH_PREC(prec, launch_copy, inp, out, g, l, seconds);
+}
+
+int sb_onesided_average(int prec, int axis, const void *inp, void *out, const sb_geom *g,
+                        const sb_launch *l, double *seconds) {
+    SB_REQUIRE(axis >= 0 && axis < 3, "axis %d not in {0, 1, 2}", axis);
+    if (check_geom(g, axis == 0, axis == 1, axis == 2)) return 2;
+    return SB_DISPATCH_PREC(prec, launch_avg1, inp, out, g, axis, l, seconds);
+}
+
+int sb_symmetric_average(int prec, int axis, const void *inp, void *out, const sb_geom *g,
+                         const sb_launch *l, double *seconds) {
+    SB_REQUIRE(axis >= 0 && axis < 3, "axis %d not in {0, 1, 2}", axis);
+    if (check_geom(g, axis == 0, axis == 1, axis == 2)) return 2;
+    return SB_DISPATCH_PREC(prec, launch_avg2, inp, out, g, axis, l, seconds);
+}
+
+int sb_laplacian(int prec, int along_x, int along_y, int along_z, const void *inp, void *out,
+                 const sb_geom *g, const sb_launch *l, double *seconds) {
+    const int mask = (along_x ? 1 : 0) | (along_y ? 2 : 0) | (along_z ? 4 : 0);
+    if (check_geom(g, along_x ? 1 : 0, along_y ? 1 : 0, along_z ? 1 : 0)) return 2;
+    return SB_DISPATCH_PREC(prec, launch_lap, inp, out, g, mask, l, seconds);
+}
+
+}  // extern "C"

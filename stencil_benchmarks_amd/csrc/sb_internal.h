@@ -1,0 +1,132 @@
+// Internal helpers shared by the libsbmi355x.so translation units: error
+// state, per-device context, launch timing and small device utilities.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/sbmi355x.h"
+
+namespace sb {
+
+// ---- error state --------------------------------------------------------
+void set_error(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+#define SB_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t sb_status_ = (call);                                       \
+        if (sb_status_ != hipSuccess) {                                       \
+            ::sb::set_error("%s failed: %s (%s:%d)", #call,                   \
+                            hipGetErrorString(sb_status_), __FILE__,          \
+                            __LINE__);                                        \
+            (void)hipGetLastError();                                          \
+            return 1;                                                         \
+        }                                                                     \
+    } while (0)
+
+#define SB_REQUIRE(cond, ...)                                                 \
+    do {                                                                      \
+        if (!(cond)) {                                                        \
+            ::sb::set_error(__VA_ARGS__);                                     \
+            return 2;                                                         \
+        }                                                                     \
+    } while (0)
+
+// ---- per-device context -------------------------------------------------
+struct Context {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    hipEvent_t start = nullptr, stop = nullptr;
+    void *flush_buffer = nullptr;
+    size_t flush_bytes = 0;
+    int compute_units = 0;
+    int xcds = 1;
+    size_t lds_per_block = 0;
+};
+
+// Context of the current device; initialises it on first use.  Returns
+// nullptr (with the error set) on failure.
+Context *context();
+
+// Evict L2 and the Infinity Cache by streaming a large scratch buffer.
+int flush_caches(Context *ctx);
+
+// Run `launch` (a callable returning hipError_t after enqueueing one
+// launch on ctx->stream) under the timing contract of the C ABI.
+template <typename Launch>
+int timed_launch(const sb_launch *l, double *seconds, Launch &&launch) {
+    Context *ctx = context();
+    if (!ctx) return 1;
+    const int dry = l ? l->dry_runs : 0;
+    if (!seconds) {
+        SB_HIP(launch());
+        return 0;
+    }
+    for (int d = 0; d < dry; ++d) SB_HIP(launch());
+    if (l && l->flush) {
+        if (flush_caches(ctx)) return 1;
+    }
+    SB_HIP(hipEventRecord(ctx->start, ctx->stream));
+    SB_HIP(launch());
+    SB_HIP(hipEventRecord(ctx->stop, ctx->stream));
+    SB_HIP(hipEventSynchronize(ctx->stop));
+    float ms = 0.f;
+    SB_HIP(hipEventElapsedTime(&ms, ctx->start, ctx->stop));
+    *seconds = double(ms) * 1e-3;
+    return 0;
+}
+
+// Validate pointer / geometry preconditions common to all stencils.
+int check_geom(const sb_geom *g, int64_t need_hi, int64_t need_hj,
+               int64_t need_hk);
+
+// Largest vector width (elements, power of two, at most 16 bytes) such that
+// every pointer in `ptrs` and the strides sj, sk are multiples of it; the
+// unit stride si must be 1, else 1 is returned.
+int vector_width(size_t elem_bytes, const void *const *ptrs, int nptrs,
+                 const sb_geom *g);
+
+// Map a linear workgroup index so that consecutive "logical" tiles land on
+// the same XCD: hardware dispatches workgroup b to XCD b % xcds.
+__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t total, uint32_t xcds) {
+    if (xcds <= 1 || total < xcds) return b;
+    const uint32_t per = total / xcds;      // full rounds
+    const uint32_t full = per * xcds;
+    if (b >= full) return b;                // ragged tail keeps its order
+    return (b % xcds) * per + b / xcds;
+}
+
+template <int P>
+struct Prec;
+template <>
+struct Prec<SB_F64> {
+    using T = double;
+    using C = double;
+};
+template <>
+struct Prec<SB_F32> {
+    using T = float;
+    using C = float;
+};
+template <>
+struct Prec<SB_F32_F64> {
+    using T = float;
+    using C = double;
+};
+
+inline size_t prec_bytes(int prec) { return prec == SB_F64 ? 8 : 4; }
+
+}  // namespace sb
+
+// Dispatch a templated functor on the runtime precision code.
+#define SB_DISPATCH_PREC(prec, FUNCTOR, ...)                                  \
+    ((prec) == SB_F64       ? FUNCTOR<SB_F64>(__VA_ARGS__)                    \
+     : (prec) == SB_F32     ? FUNCTOR<SB_F32>(__VA_ARGS__)                    \
+     : (prec) == SB_F32_F64 ? FUNCTOR<SB_F32_F64>(__VA_ARGS__)                \
+                            : (::sb::set_error("invalid precision code %d",  \
+                                               (int)(prec)),                  \
+                               2))

@@ -1,0 +1,294 @@
+// STREAM copy / scale / add / triad on HBM-resident arrays.
+//
+// Each thread moves U vectors of V elements (16 bytes per vector, so one
+// global_load/store_dwordx4 each); the U vectors of a thread are one block
+// width apart so every load instruction of a wave covers 1 KiB contiguously.
+// Optional non-temporal hints keep the streamed lines from displacing L2.
+#include "sb_internal.h"
+
+namespace sb {
+namespace {
+
+template <typename T, int V>
+using vec_t = T __attribute__((ext_vector_type(V)));
+
+template <bool NT, typename VT>
+__device__ __forceinline__ VT load(const VT *p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <bool NT, typename VT>
+__device__ __forceinline__ void store(VT *p, VT v) {
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+// c = a; b = s c; c = a + b; a = b + s c  (stream/cuda_hip.j2:147-173)
+template <typename T, int OP, int V, int U, bool NTL, bool NTS>
+__global__ __launch_bounds__(1024) void stream_kernel(T *__restrict__ a, T *__restrict__ b,
+                                                      T *__restrict__ c, T s, size_t nvec) {
+    using VT = vec_t<T, V>;
+    VT *A = reinterpret_cast<VT *>(a);
+    VT *B = reinterpret_cast<VT *>(b);
+    VT *Cv = reinterpret_cast<VT *>(c);
+    const size_t base = size_t(blockIdx.x) * (size_t(U) * blockDim.x) + threadIdx.x;
+    VT x[U], y[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const size_t idx = base + size_t(u) * blockDim.x;
+        if (idx < nvec) {
+            if constexpr (OP == SB_STREAM_COPY) {
+                x[u] = load<NTL>(A + idx);
+            } else if constexpr (OP == SB_STREAM_SCALE) {
+                x[u] = load<NTL>(Cv + idx);
+            } else if constexpr (OP == SB_STREAM_ADD) {
+                x[u] = load<NTL>(A + idx);
+                y[u] = load<NTL>(B + idx);
+            } else {
+                x[u] = load<NTL>(B + idx);
+                y[u] = load<NTL>(Cv + idx);
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const size_t idx = base + size_t(u) * blockDim.x;
+        if (idx < nvec) {
+            if constexpr (OP == SB_STREAM_COPY) {
+                store<NTS>(Cv + idx, x[u]);
+            } else if constexpr (OP == SB_STREAM_SCALE) {
+                store<NTS>(B + idx, s * x[u]);
+            } else if constexpr (OP == SB_STREAM_ADD) {
+                store<NTS>(Cv + idx, x[u] + y[u]);
+            } else {
+                store<NTS>(A + idx, x[u] + s * y[u]);
+            }
+        }
+    }
+}
+
+// Elements past the last full vector.
+template <typename T, int OP>
+__global__ void stream_tail(T *a, T *b, T *c, T s, size_t first, size_t n) {
+    const size_t i = first + threadIdx.x;
+    if (i >= n) return;
+    if constexpr (OP == SB_STREAM_COPY)
+        c[i] = a[i];
+    else if constexpr (OP == SB_STREAM_SCALE)
+        b[i] = s * c[i];
+    else if constexpr (OP == SB_STREAM_ADD)
+        c[i] = a[i] + b[i];
+    else
+        a[i] = b[i] + s * c[i];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_kernel(T *x, size_t n, T value) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x)
+        x[i] = value;
+}
+
+// splitmix64 finaliser of (seed, index): uniform [0,1) with 53 (double) or
+// 24 (float) random mantissa bits.
+__device__ __host__ inline uint64_t mix64(uint64_t seed, uint64_t i) {
+    uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_random_kernel(T *x, size_t n, uint64_t seed) {
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const uint64_t z = mix64(seed, i);
+        if constexpr (sizeof(T) == 8)
+            x[i] = T(double(z >> 11) * 0x1.0p-53);
+        else
+            x[i] = T(float(uint32_t(z >> 40)) * 0x1.0p-24f);
+    }
+}
+
+// Max relative deviation from `expected`, as the bit pattern of a
+// non-negative double (monotonic under unsigned comparison).
+template <typename T>
+__global__ __launch_bounds__(256) void check_kernel(const T *x, size_t n, double expected,
+                                                    unsigned long long *result) {
+    double worst = 0.0;
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const double rel = fabs((double(x[i]) - expected) / expected);
+        worst = rel > worst ? rel : worst;
+    }
+    for (int off = 32; off > 0; off /= 2) {
+        const double other = __shfl_down(worst, off);
+        worst = other > worst ? other : worst;
+    }
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(result, (unsigned long long)__double_as_longlong(worst));
+}
+
+template <typename T, int OP, int V, int U>
+int launch_stream(T *a, T *b, T *c, size_t n, double scalar, const sb_launch *l,
+                  double *seconds, int block) {
+    const size_t nvec = n / V;
+    const size_t per_block = size_t(U) * block;
+    const size_t blocks = (nvec + per_block - 1) / per_block;
+    const int nt = l ? (l->variant & 3) : 0;
+    Context *ctx = context();
+    if (!ctx) return 1;
+    const T s = T(scalar);
+    return timed_launch(l, seconds, [&]() -> hipError_t {
+        if (blocks) {
+            switch (nt) {
+                case 0:
+                    hipLaunchKernelGGL((stream_kernel<T, OP, V, U, false, false>), dim3(blocks),
+                                       dim3(block), 0, ctx->stream, a, b, c, s, nvec);
+                    break;
+                case 1:
+                    hipLaunchKernelGGL((stream_kernel<T, OP, V, U, true, false>), dim3(blocks),
+                                       dim3(block), 0, ctx->stream, a, b, c, s, nvec);
+                    break;
+                case 2:
+                    hipLaunchKernelGGL((stream_kernel<T, OP, V, U, false, true>), dim3(blocks),
+                                       dim3(block), 0, ctx->stream, a, b, c, s, nvec);
+                    break;
+                default:
+                    hipLaunchKernelGGL((stream_kernel<T, OP, V, U, true, true>), dim3(blocks),
+                                       dim3(block), 0, ctx->stream, a, b, c, s, nvec);
+                    break;
+            }
+        }
+        if (nvec * V < n)
+            hipLaunchKernelGGL((stream_tail<T, OP>), dim3(1), dim3(V), 0, ctx->stream, a, b, c,
+                               s, nvec * V, n);
+        return hipGetLastError();
+    });
+}
+
+template <typename T, int OP>
+int dispatch_unroll(T *a, T *b, T *c, size_t n, double scalar, const sb_launch *l,
+                    double *seconds) {
+    constexpr int V = 16 / sizeof(T);
+    const int block = (l && l->bx) ? l->bx : 256;
+    SB_REQUIRE(block >= 64 && block <= 1024 && block % 64 == 0,
+               "STREAM block size %d must be a multiple of 64 in [64, 1024]", block);
+    const int unroll = (l && l->unroll) ? l->unroll : 4;
+    switch (unroll) {
+        case 1: return launch_stream<T, OP, V, 1>(a, b, c, n, scalar, l, seconds, block);
+        case 2: return launch_stream<T, OP, V, 2>(a, b, c, n, scalar, l, seconds, block);
+        case 4: return launch_stream<T, OP, V, 4>(a, b, c, n, scalar, l, seconds, block);
+        case 8: return launch_stream<T, OP, V, 8>(a, b, c, n, scalar, l, seconds, block);
+        default: set_error("STREAM unroll %d not in {1, 2, 4, 8}", unroll); return 2;
+    }
+}
+
+template <typename T>
+int dispatch_op(int op, T *a, T *b, T *c, size_t n, double scalar, const sb_launch *l,
+                double *seconds) {
+    for (T *p : {a, b, c})
+        SB_REQUIRE(reinterpret_cast<uintptr_t>(p) % 16 == 0,
+                   "STREAM arrays must be 16-byte aligned");
+    switch (op) {
+        case SB_STREAM_COPY: return dispatch_unroll<T, SB_STREAM_COPY>(a, b, c, n, scalar, l, seconds);
+        case SB_STREAM_SCALE: return dispatch_unroll<T, SB_STREAM_SCALE>(a, b, c, n, scalar, l, seconds);
+        case SB_STREAM_ADD: return dispatch_unroll<T, SB_STREAM_ADD>(a, b, c, n, scalar, l, seconds);
+        case SB_STREAM_TRIAD: return dispatch_unroll<T, SB_STREAM_TRIAD>(a, b, c, n, scalar, l, seconds);
+        default: set_error("invalid STREAM kernel %d", op); return 2;
+    }
+}
+
+int grid_for(size_t n, Context *ctx) {
+    const size_t want = (n + 255) / 256;
+    const size_t cap = size_t(ctx->compute_units) * 32;
+    return int(want < cap ? (want ? want : 1) : cap);
+}
+
+}  // namespace
+}  // namespace sb
+
+using namespace sb;
+
+extern "C" {
+
+int sb_stream_fill(int prec, void *x, size_t n, double value) {
+    SB_REQUIRE(prec == SB_F64 || prec == SB_F32, "STREAM precision must be SB_F64 or SB_F32");
+    SB_REQUIRE(x || !n, "NULL array");
+    Context *ctx = context();
+    if (!ctx || !n) return ctx ? 0 : 1;
+    if (prec == SB_F64)
+        hipLaunchKernelGGL(fill_kernel<double>, dim3(grid_for(n, ctx)), dim3(256), 0, ctx->stream,
+                           static_cast<double *>(x), n, value);
+    else
+        hipLaunchKernelGGL(fill_kernel<float>, dim3(grid_for(n, ctx)), dim3(256), 0, ctx->stream,
+                           static_cast<float *>(x), n, float(value));
+    SB_HIP(hipGetLastError());
+    SB_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int sb_stream_fill_random(int prec, void *x, size_t n, uint64_t seed) {
+    SB_REQUIRE(prec == SB_F64 || prec == SB_F32, "STREAM precision must be SB_F64 or SB_F32");
+    SB_REQUIRE(x || !n, "NULL array");
+    Context *ctx = context();
+    if (!ctx || !n) return ctx ? 0 : 1;
+    if (prec == SB_F64)
+        hipLaunchKernelGGL(fill_random_kernel<double>, dim3(grid_for(n, ctx)), dim3(256), 0,
+                           ctx->stream, static_cast<double *>(x), n, seed);
+    else
+        hipLaunchKernelGGL(fill_random_kernel<float>, dim3(grid_for(n, ctx)), dim3(256), 0,
+                           ctx->stream, static_cast<float *>(x), n, seed);
+    SB_HIP(hipGetLastError());
+    SB_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int sb_stream_kernel(int prec, int op, void *a, void *b, void *c, size_t n, double scalar,
+                     const sb_launch *l, double *seconds) {
+    SB_REQUIRE(a && b && c, "NULL STREAM array");
+    if (prec == SB_F64)
+        return dispatch_op<double>(op, static_cast<double *>(a), static_cast<double *>(b),
+                                   static_cast<double *>(c), n, scalar, l, seconds);
+    if (prec == SB_F32)
+        return dispatch_op<float>(op, static_cast<float *>(a), static_cast<float *>(b),
+                                  static_cast<float *>(c), n, scalar, l, seconds);
+    set_error("STREAM precision must be SB_F64 or SB_F32");
+    return 2;
+}
+
+int sb_stream_check(int prec, const void *x, size_t n, double expected, double *max_rel_err) {
+    SB_REQUIRE(prec == SB_F64 || prec == SB_F32, "STREAM precision must be SB_F64 or SB_F32");
+    SB_REQUIRE(max_rel_err, "NULL result pointer");
+    SB_REQUIRE(expected != 0.0, "expected value must be non-zero");
+    Context *ctx = context();
+    if (!ctx) return 1;
+    unsigned long long *d_res = nullptr;
+    SB_HIP(hipMallocAsync(reinterpret_cast<void **>(&d_res), sizeof(*d_res), ctx->stream));
+    SB_HIP(hipMemsetAsync(d_res, 0, sizeof(*d_res), ctx->stream));
+    if (n) {
+        if (prec == SB_F64)
+            hipLaunchKernelGGL(check_kernel<double>, dim3(grid_for(n, ctx)), dim3(256), 0,
+                               ctx->stream, static_cast<const double *>(x), n, expected, d_res);
+        else
+            hipLaunchKernelGGL(check_kernel<float>, dim3(grid_for(n, ctx)), dim3(256), 0,
+                               ctx->stream, static_cast<const float *>(x), n, expected, d_res);
+        SB_HIP(hipGetLastError());
+    }
+    unsigned long long bits = 0;
+    SB_HIP(hipMemcpyAsync(&bits, d_res, sizeof(bits), hipMemcpyDeviceToHost, ctx->stream));
+    SB_HIP(hipFreeAsync(d_res, ctx->stream));
+    SB_HIP(hipStreamSynchronize(ctx->stream));
+    double v;
+    static_assert(sizeof(v) == sizeof(bits), "size");
+    __builtin_memcpy(&v, &bits, sizeof(v));
+    *max_rel_err = v;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,250 @@
+"""HIP kernels vs the oracle, through the C ABI, on seeded inputs.
+
+Bar: FP64 results equal the oracle's bit for bit (the kernels follow the
+reference oracle's operation order with no FMA contraction).  Float32
+storage with float32 arithmetic equals the float32 oracle bit for bit.
+Float32 storage with float64 arithmetic equals the float64 oracle run on
+the same (float32-rounded) inputs, rounded once to float32, bit for bit.
+Every case also checks that no point outside the interior of the output
+was written (sentinel halo).
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import inner, make_fields
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+SENTINEL = -1234.5
+PRECISIONS = ["f64", "f32", "f32_f64"]
+
+
+def _prec(native, name):
+    return {"f64": native.SB_F64, "f32": native.SB_F32, "f32_f64": native.SB_F32_F64}[name]
+
+
+def _dtype(name):
+    return np.float64 if name == "f64" else np.float32
+
+
+def _expected(func, fields, prec, *args):
+    """Oracle result in the storage type for precision `prec`."""
+    if prec == "f32_f64":
+        widened = [f.astype(np.float64) for f in fields]
+        from stencil_benchmarks_amd.tools import array
+
+        clones = []
+        for f, w in zip(fields, widened):
+            c = array.alloc_array(f.shape, np.float64, _layout_of(f), 0, (0, 0, 0))
+            c[...] = w
+            clones.append(c)
+        return func(*clones, *args).astype(np.float32)
+    return func(*fields, *args)
+
+
+def _layout_of(a):
+    order = np.argsort(-np.array(a.strides))
+    layout = [0, 0, 0]
+    for rank, axis in enumerate(order):
+        layout[axis] = rank
+    return tuple(layout)
+
+
+def _run(native, entry, fields, written, domain, halo, *pre_args, launch=None):
+    from stencil_benchmarks_amd.device import DeviceField, geometry
+
+    devs = [DeviceField(f, halo) for f in fields]
+    for d, f in zip(devs, fields):
+        d.upload(f)
+    geom = geometry(domain, [s // fields[0].itemsize for s in fields[0].strides], halo)
+    ptrs = [ctypes.c_void_p(d.pointer()) for d in devs]
+    opts = launch or native.launch_options()
+    native.call(entry, *pre_args, *ptrs, geom, opts, None)
+    native.call("sb_synchronize")
+    results = []
+    for w in written:
+        out = np.ndarray(fields[w].shape, fields[w].dtype,
+                         buffer=np.zeros(devs[w].span, np.uint8), strides=fields[w].strides)
+        devs[w].download(out)
+        results.append(out)
+    return results
+
+
+def _with_sentinel(field):
+    field[...] = SENTINEL
+    return field
+
+
+def _check_halo(out, domain, halo):
+    mask = np.ones(out.shape, dtype=bool)
+    mask[inner(domain, halo)] = False
+    assert np.all(out[mask] == SENTINEL), "kernel wrote outside the interior"
+
+
+BASIC_DOMAINS = [(64, 16, 4), (37, 13, 5), (1, 1, 1), (130, 3, 2)]
+GEOMETRIES = [((2, 1, 0), 0), ((2, 1, 0), 64), ((0, 1, 2), 0), ((1, 2, 0), 16)]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("layout,alignment", GEOMETRIES)
+@pytest.mark.parametrize("domain", BASIC_DOMAINS)
+def test_basic_stencils(gpu, prec, layout, alignment, domain):
+    halo = (1, 2, 1)
+    dtype = _dtype(prec)
+    code = _prec(gpu, prec)
+    inp, = make_fields(domain, halo, dtype, layout, alignment, seed=hash(domain) % 1000)
+    cases = [("sb_copy", (), lambda f: oracle.copy(f, domain, halo))]
+    for axis in range(3):
+        cases.append(("sb_onesided_average", (axis,),
+                      lambda f, a=axis: oracle.onesided_average(f, a, domain, halo)))
+        cases.append(("sb_symmetric_average", (axis,),
+                      lambda f, a=axis: oracle.symmetric_average(f, a, domain, halo)))
+    for mask in [(1, 1, 0), (1, 0, 0), (0, 1, 1), (1, 1, 1)]:
+        cases.append(("sb_laplacian", mask,
+                      lambda f, m=mask: oracle.laplacian(f, m, domain, halo)))
+    for entry, extra, func in cases:
+        out = _with_sentinel(make_fields(domain, halo, dtype, layout, alignment)[0])
+        got, = _run(gpu, entry, [inp, out], [1], domain, halo, code, *extra)
+        expected = _expected(func, [inp], prec)
+        assert np.array_equal(got[inner(domain, halo)], expected), (entry, extra)
+        _check_halo(got, domain, halo)
+
+
+HDIFF_DOMAINS = [(70, 37, 9), (64, 16, 1), (1, 1, 1), (129, 33, 3), (5, 70, 2)]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("variant,tile", [(1, 8), (1, 16), (1, 32), (3, 0)])
+@pytest.mark.parametrize("domain", HDIFF_DOMAINS)
+def test_hdiff(gpu, prec, variant, tile, domain):
+    halo = (2, 3, 0)
+    dtype = _dtype(prec)
+    inp, coeff = make_fields(domain, halo, dtype, count=2, seed=sum(domain))
+    out = _with_sentinel(make_fields(domain, halo, dtype)[0])
+    opts = gpu.launch_options(variant=variant, block=(64, tile, 1))
+    got, = _run(gpu, "sb_hdiff", [inp, coeff, out], [2], domain, halo, _prec(gpu, prec),
+                launch=opts)
+    expected = _expected(lambda a, b: oracle.hdiff(a, b, domain, halo), [inp, coeff], prec)
+    assert np.array_equal(got[inner(domain, halo)], expected)
+    _check_halo(got, domain, halo)
+
+
+@pytest.mark.parametrize("layout,alignment", [((0, 1, 2), 0), ((1, 0, 2), 8)])
+def test_hdiff_other_layouts(gpu, layout, alignment):
+    domain, halo = (33, 21, 7), (3, 3, 3)
+    inp, coeff = make_fields(domain, halo, np.float64, layout, alignment, count=2)
+    out = _with_sentinel(make_fields(domain, halo, np.float64, layout, alignment)[0])
+    got, = _run(gpu, "sb_hdiff", [inp, coeff, out], [2], domain, halo, gpu.SB_F64)
+    assert np.array_equal(got[inner(domain, halo)], oracle.hdiff(inp, coeff, domain, halo))
+    _check_halo(got, domain, halo)
+
+
+VADV_DOMAINS = [(70, 37, 9), (64, 1, 2), (1, 1, 2), (65, 3, 33)]
+
+
+def _vadv_expected(fields, domain, halo, prec, components):
+    ncomp = len(components)
+    wcon = fields[4 * ncomp]
+    out = []
+    for q, comp in enumerate(components):
+        f = fields[4 * q: 4 * q + 4] + [wcon]
+        shift = oracle.COMPONENT_SHIFTS[comp]
+        out.append(_expected(lambda *a: oracle.vadv(*a, domain, halo, shift), f, prec))
+    return out
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("all_components", [False, True])
+@pytest.mark.parametrize("domain", VADV_DOMAINS)
+def test_vadv(gpu, prec, variant, all_components, domain):
+    halo = (1, 1, 0)
+    dtype = _dtype(prec)
+    components = ("u", "v", "w") if all_components else ("u",)
+    n = 4 * len(components) + 1
+    fields = make_fields(domain, halo, dtype, count=n, seed=7 + sum(domain))
+    scratch = make_fields(domain, halo, dtype, count=3)
+    written = [4 * q + 3 for q in range(len(components))]
+    got = _run_vadv(gpu, fields + scratch, written, domain, halo, _prec(gpu, prec),
+                    all_components, variant)
+    expected = _vadv_expected(fields, domain, halo, prec, components)
+    for g, e in zip(got, expected):
+        assert np.array_equal(g[inner(domain, halo)], e)
+
+
+def _run_vadv(native, fields, written, domain, halo, code, all_components, variant):
+    from stencil_benchmarks_amd.device import DeviceField, geometry
+
+    devs = [DeviceField(f, halo) for f in fields]
+    for d, f in zip(devs, fields):
+        d.upload(f)
+    geom = geometry(domain, [s // fields[0].itemsize for s in fields[0].strides], halo)
+    array = (ctypes.c_void_p * len(devs))(*[d.pointer() for d in devs])
+    native.call("sb_vadv", code, int(all_components), array, geom,
+                native.launch_options(variant=variant), None)
+    native.call("sb_synchronize")
+    results = []
+    for w in written:
+        out = np.ndarray(fields[w].shape, fields[w].dtype,
+                         buffer=np.zeros(devs[w].span, np.uint8), strides=fields[w].strides)
+        devs[w].download(out)
+        results.append(out)
+    return results
+
+
+def test_vadv_tall_column_uses_global_scratch(gpu):
+    # nz = 161 exceeds the LDS budget of the FP64 column kernel; variant 0
+    # must fall back to the global scratch and still match.
+    domain, halo = (67, 2, 161), (1, 0, 0)
+    fields = make_fields(domain, halo, np.float64, count=8, seed=3)
+    got, = _run_vadv(gpu, fields, [3], domain, halo, gpu.SB_F64, False, 0)
+    expected = oracle.vadv(*fields[:5], domain, halo)
+    assert np.array_equal(got[inner(domain, halo)], expected)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("op", ["copy", "scale", "add", "triad"])
+@pytest.mark.parametrize("unroll,nt", [(1, 0), (4, 3), (8, 1)])
+def test_stream_kernels(gpu, dtype, op, unroll, nt):
+    from stencil_benchmarks_amd.device import DeviceBuffer
+
+    n = 100003
+    code = gpu.SB_F64 if dtype == "float64" else gpu.SB_F32
+    rng = np.random.default_rng(5)
+    host = [rng.random(n).astype(dtype) for _ in range(3)]
+    bufs = [DeviceBuffer(n * host[0].itemsize) for _ in range(3)]
+    for b, h in zip(bufs, host):
+        gpu.call("sb_memcpy_h2d", ctypes.c_void_p(b.ptr), ctypes.c_void_p(h.ctypes.data), h.nbytes)
+    opts = gpu.launch_options(variant=nt, block=(256, 1, 1), unroll=unroll)
+    gpu.call("sb_stream_kernel", code, gpu.STREAM_OPS[op], *[ctypes.c_void_p(b.ptr) for b in bufs],
+             n, 3.0, opts, None)
+    gpu.call("sb_synchronize")
+    got = []
+    for b, h in zip(bufs, host):
+        out = np.empty_like(h)
+        gpu.call("sb_memcpy_d2h", ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(b.ptr), h.nbytes)
+        got.append(out)
+    a, b, c = host
+    s = np.dtype(dtype).type(3.0)
+    expected = {"copy": (a, b, a), "scale": (a, s * c, c), "add": (a, b, a + b),
+                "triad": (b + s * c, b, c)}[op]
+    for g, e in zip(got, expected):
+        assert np.array_equal(g, e)
+
+
+def test_stream_random_fill_matches_host_hash(gpu):
+    from stencil_benchmarks_amd.device import DeviceBuffer
+    from stencil_benchmarks_amd.tools.hashfill import uniform
+
+    n = 4099
+    for dtype, code in (("float64", gpu.SB_F64), ("float32", gpu.SB_F32)):
+        buf = DeviceBuffer(n * np.dtype(dtype).itemsize)
+        gpu.call("sb_stream_fill_random", code, ctypes.c_void_p(buf.ptr), n, 11)
+        out = np.empty(n, dtype)
+        gpu.call("sb_memcpy_d2h", ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(buf.ptr), out.nbytes)
+        assert np.array_equal(out, uniform(11, np.arange(n), dtype))
+        assert out.min() >= 0 and out.max() < 1
