@@ -17,13 +17,17 @@ from .mixin import StencilMixin
 
 
 class HorizontalDiffusionMixin(StencilMixin):
-    block_size = Parameter("tile size (i fixed to 64; j in 8, 16, 32)", (64, 16, 1))
+    block_size = Parameter(
+        "tile size: i (64), j (8, 16, 32), k-planes per workgroup (1, 2, 4, 8, 16)",
+        (64, 16, 8),
+    )
 
     VARIANT = 1
 
     def setup(self):
         super().setup()
-        if self.VARIANT == 1 and (self.block_size[0] != 64 or self.block_size[1] not in (8, 16, 32)):
+        bx, by, bz = self.block_size
+        if self.VARIANT == 1 and (bx != 64 or by not in (8, 16, 32) or bz not in (1, 2, 4, 8, 16)):
             raise ParameterError(f"unsupported tile size {self.block_size}")
 
     @property
@@ -33,7 +37,7 @@ class HorizontalDiffusionMixin(StencilMixin):
     def launch_options(self):
         options = super().launch_options()
         options.variant = self.variant or self.VARIANT
-        options.bx, options.by = self.block_size[0], self.block_size[1]
+        options.bx, options.by, options.bz = self.block_size
         return options
 
     def launch(self, pointers, options, seconds):

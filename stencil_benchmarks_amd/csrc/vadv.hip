@@ -14,22 +14,31 @@
 //                tensstage(k) = dtr (x(k) - pos(k))
 // with dtr = 3/20 and bet_m = bet_p = 0.5.
 //
-// One lane owns one column; a workgroup is one wave of 64 consecutive i at
-// fixed j, so every field access of a wave is one contiguous 64-element row
-// segment.  The forward sweep streams the inputs PF levels ahead of the
-// recurrence (the loads do not depend on it), keeps stage/wcon of the
-// previous level in registers, and stores (c', d') per level either in LDS
-// (variant 1: nz x 64 x 16 bytes per wave for FP64 arithmetic) or in a
-// library-owned FP64 global scratch laid out [k][column] (variant 2).  The
-// backward substitution runs in the same kernel from that storage.  All
-// arithmetic is in C (double for SB_F64 and SB_F32_F64); the ccol, dcol and
-// datacol fields of the reference's argument list are not used.
+// Work decomposition.  One lane owns one column; the 64 lanes of the
+// compute wave own 64 consecutive i at fixed j, so every field access of a
+// level is one contiguous 64-element row segment.  The compute wave keeps
+// stage / wcon of the previous level in registers, loads the next PF levels
+// while it computes the current ones, stores (c', d') of levels 0..nz-2 and
+// runs the backward substitution from that storage in the same kernel.
+//
+// Storage of (c', d'): variant 1 keeps it in LDS ((nz-1) x 64 x 16 bytes
+// for FP64 arithmetic, so nz <= 160 fits one workgroup); variant 2 keeps it
+// in a library-owned FP64 global scratch laid out [k][column].
+//
+// Memory-level parallelism.  With LDS holding the coefficients, a CU runs a
+// single 64-column compute wave at nz = 160 and one wave can only track 63
+// outstanding loads, far less than HBM latency needs.  Variant 1 therefore
+// pairs every compute wave with a helper wave in the same workgroup that
+// walks LEAD levels ahead and touches each cache line of the coming levels
+// (one 4-byte load per 64-byte half line, results folded into a dummy
+// sum), so the compute wave's own loads hit L2.  The helper paces itself on
+// the compute wave's progress counter in LDS with bounded waits (it never
+// blocks the compute wave), and after the forward sweep it does the same
+// for the pos lines of the backward substitution.
 #include "sb_internal.h"
 
 namespace sb {
 namespace {
-
-constexpr int PF = 8;  // prefetch depth in levels
 
 struct VadvArgs {
     int64_t nx, ny, nz, si, sj, sk;
@@ -53,127 +62,253 @@ struct alignas(2 * sizeof(C)) CD {
     C c, d;
 };
 
-template <typename C>
+// Inputs of one interior level k: pos, tens, tensstage at k and stage,
+// wcon, wcon neighbour at k + 1 (stored in the field type T).
+template <typename T>
 struct Level {
-    C pos, tens, ts;  // at k
-    C st, w, wn;      // stage, wcon, wcon neighbour at k+1
+    T pos, tens, ts;
+    T st, w, wn;
 };
 
-template <typename T, typename C, bool LDS>
+constexpr int PB = 16;        // backward-sweep prefetch depth (levels)
+constexpr int LEAD = 32;      // helper lead over the compute wave (levels)
+constexpr int MAX_SPINS = 4096;
+
+// Progress word shared by the compute and helper waves of a workgroup:
+// forward level k published as k, backward level k as 2 nz - 1 - k.
+__device__ __forceinline__ void publish(int *progress, int value) {
+    if (progress) __hip_atomic_store(progress, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ int observe(const int *progress) {
+    return __hip_atomic_load(progress, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Wait (bounded) until the compute wave has published at least `target`.
+__device__ __forceinline__ void pace(const int *progress, int target) {
+    for (int spin = 0; spin < MAX_SPINS && observe(progress) < target; ++spin)
+        __builtin_amdgcn_s_sleep(4);
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ T stream_load(const T *p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <typename T, typename C, int PF, bool NT>
 __device__ __forceinline__ void solve_column(const Component &cp, const T *__restrict__ wcon,
                                              int64_t base, int64_t sk, int nz, CD<C> *scratch,
-                                             int64_t sstride, bool active) {
-    const T *__restrict__ stage = static_cast<const T *>(cp.stage);
-    const T *__restrict__ pos = static_cast<const T *>(cp.pos);
-    const T *__restrict__ tens = static_cast<const T *>(cp.tens);
-    T *__restrict__ tstage = static_cast<T *>(cp.tensstage);
-    const T *__restrict__ wn = wcon + cp.wshift;
+                                             int64_t sstride, bool active, int *progress) {
+    const T *__restrict__ stage = static_cast<const T *>(cp.stage) + base;
+    const T *__restrict__ pos = static_cast<const T *>(cp.pos) + base;
+    const T *__restrict__ tens = static_cast<const T *>(cp.tens) + base;
+    T *__restrict__ tstage = static_cast<T *>(cp.tensstage) + base;
+    const T *__restrict__ w0 = wcon + base;
+    const T *__restrict__ wn0 = wcon + base + cp.wshift;
 
     const C dtr = C(3.0 / 20.0);
     const C bet_m = C(0.5), bet_p = C(0.5);
+    const int kmid_last = nz - 2;  // last interior level with a k+1 neighbour
 
-    auto load = [&](int k, Level<C> &r) {
-        const int64_t p = base + int64_t(k) * sk;
-        r.pos = C(pos[p]);
-        r.tens = C(tens[p]);
-        r.ts = C(tstage[p]);
-        if (k + 1 < nz) {
-            r.st = C(stage[p + sk]);
-            r.w = C(wcon[p + sk]);
-            r.wn = C(wn[p + sk]);
-        } else {
-            r.st = r.w = r.wn = C(0);
-        }
+    // Level record of k; callers clamp k into [0, nz-2] so every load is in
+    // bounds (records past the end are re-reads of level nz-2, discarded).
+    auto load = [&](int k, Level<T> &r) {
+        const int64_t p = int64_t(k) * sk;
+        r.pos = pos[p];
+        r.tens = stream_load<NT>(tens + p);
+        r.ts = stream_load<NT>(tstage + p);
+        r.st = stream_load<NT>(stage + p + sk);
+        r.w = stream_load<NT>(w0 + p + sk);
+        r.wn = stream_load<NT>(wn0 + p + sk);
     };
 
-    C st_m = C(0), st_c = C(stage[base]), w_c = C(wcon[base]), wn_c = C(wn[base]);
-    C c_prev = C(0), d_prev = C(0);
+    // ---- k = 0 ----------------------------------------------------------
+    Level<T> first;
+    load(0, first);
+    Level<T> cur[PF], nxt[PF];
+#pragma unroll
+    for (int p = 0; p < PF; ++p) load(min(1 + p, kmid_last), cur[p]);
+    const C st0 = C(stage[0]);
 
-    Level<C> cur[PF] = {}, nxt[PF] = {};
-#pragma unroll
-    for (int p = 0; p < PF; ++p)
-        if (p < nz) load(p, cur[p]);
-
-    for (int kb = 0; kb < nz; kb += PF) {
-#pragma unroll
-        for (int p = 0; p < PF; ++p)
-            if (kb + PF + p < nz) load(kb + PF + p, nxt[p]);
-#pragma unroll
-        for (int p = 0; p < PF; ++p) {
-            const int k = kb + p;
-            if (k >= nz) break;
-            const Level<C> &r = cur[p];
-            C cc, dd;
-            if (k == 0) {
-                const C gcv = C(0.25) * (r.wn + r.w);
-                const C cs = gcv * bet_m;
-                cc = gcv * bet_p;
-                const C bcol = dtr - cc;
-                const C corr = -cs * (r.st - st_c);
-                dd = ((dtr * r.pos + r.tens) + r.ts) + corr;
-                cc = cc / bcol;
-                dd = dd / bcol;
-            } else if (k < nz - 1) {
-                const C gav = C(-0.25) * (wn_c + w_c);
-                const C gcv = C(0.25) * (r.wn + r.w);
-                const C as = gav * bet_m;
-                const C cs = gcv * bet_m;
-                const C acol = gav * bet_p;
-                cc = gcv * bet_p;
-                const C bcol = (dtr - acol) - cc;
-                const C corr = -as * (st_m - st_c) - cs * (r.st - st_c);
-                dd = ((dtr * r.pos + r.tens) + r.ts) + corr;
-                const C divided = C(1.0) / (bcol - c_prev * acol);
-                cc = cc * divided;
-                dd = (dd - d_prev * acol) * divided;
-            } else {
-                const C gav = C(-0.25) * (wn_c + w_c);
-                const C as = gav * bet_m;
-                const C acol = gav * bet_p;
-                const C bcol = dtr - acol;
-                const C corr = -as * (st_m - st_c);
-                dd = ((dtr * r.pos + r.tens) + r.ts) + corr;
-                dd = (dd - d_prev * acol) / (bcol - c_prev * acol);
-                cc = C(0);
-            }
-            scratch[int64_t(k) * sstride] = CD<C>{cc, dd};
-            c_prev = cc;
-            d_prev = dd;
-            st_m = st_c;
-            st_c = r.st;
-            w_c = r.w;
-            wn_c = r.wn;
-        }
-#pragma unroll
-        for (int p = 0; p < PF; ++p) cur[p] = nxt[p];
+    C st_m, st_c, w_c, wn_c, c_prev, d_prev;
+    {
+        const C gcv = C(0.25) * (C(first.wn) + C(first.w));
+        const C cs = gcv * bet_m;
+        C cc = gcv * bet_p;
+        const C bcol = dtr - cc;
+        const C corr = -cs * (C(first.st) - st0);
+        C dd = ((dtr * C(first.pos) + C(first.tens)) + C(first.ts)) + corr;
+        cc = cc / bcol;
+        dd = dd / bcol;
+        scratch[0] = CD<C>{cc, dd};
+        c_prev = cc;
+        d_prev = dd;
+        st_m = st0;
+        st_c = C(first.st);
+        w_c = C(first.w);
+        wn_c = C(first.wn);
     }
 
-    // Backward substitution, PF levels of pos fetched ahead of use.
-    C x = C(0);
-    for (int kt = nz - 1; kt >= 0; kt -= PF) {
-        C pk[PF];
+    auto middle = [&](int k, const Level<T> &r) {
+        const C gav = C(-0.25) * (wn_c + w_c);
+        const C w_n = C(r.w), wn_n = C(r.wn), st_n = C(r.st);
+        const C gcv = C(0.25) * (wn_n + w_n);
+        const C as = gav * bet_m;
+        const C cs = gcv * bet_m;
+        const C acol = gav * bet_p;
+        C cc = gcv * bet_p;
+        const C bcol = (dtr - acol) - cc;
+        const C corr = -as * (st_m - st_c) - cs * (st_n - st_c);
+        C dd = ((dtr * C(r.pos) + C(r.tens)) + C(r.ts)) + corr;
+        const C divided = C(1.0) / (bcol - c_prev * acol);
+        cc = cc * divided;
+        dd = (dd - d_prev * acol) * divided;
+        scratch[int64_t(k) * sstride] = CD<C>{cc, dd};
+        c_prev = cc;
+        d_prev = dd;
+        st_m = st_c;
+        st_c = st_n;
+        w_c = w_n;
+        wn_c = wn_n;
+    };
+
+    // ---- 0 < k < nz-1: full blocks of PF levels, next block in flight -----
+    int kb = 1;
+    for (; kb + PF - 1 <= kmid_last; kb += PF) {
 #pragma unroll
-        for (int p = 0; p < PF; ++p)
-            pk[p] = (kt - p >= 0) ? C(pos[base + int64_t(kt - p) * sk]) : C(0);
+        for (int p = 0; p < PF; ++p) load(min(kb + PF + p, kmid_last), nxt[p]);
 #pragma unroll
-        for (int p = 0; p < PF; ++p) {
+        for (int p = 0; p < PF; ++p) middle(kb + p, cur[p]);
+#pragma unroll
+        for (int p = 0; p < PF; ++p) cur[p] = nxt[p];
+        publish(progress, kb + PF);
+    }
+    // last level's own inputs, issued before the partial block
+    const int kl = nz - 1;
+    const T last_pos = pos[int64_t(kl) * sk], last_tens = tens[int64_t(kl) * sk],
+            last_ts = tstage[int64_t(kl) * sk];
+#pragma unroll
+    for (int p = 0; p < PF; ++p)
+        if (kb + p <= kmid_last) middle(kb + p, cur[p]);
+
+    // ---- k = nz-1 (kept in registers, never stored) -----------------------
+    C x;
+    {
+        const C gav = C(-0.25) * (wn_c + w_c);
+        const C as = gav * bet_m;
+        const C acol = gav * bet_p;
+        const C bcol = dtr - acol;
+        const C corr = -as * (st_m - st_c);
+        C dd = ((dtr * C(last_pos) + C(last_tens)) + C(last_ts)) + corr;
+        x = (dd - d_prev * acol) / (bcol - c_prev * acol);
+    }
+    publish(progress, nz);
+    if (active) tstage[int64_t(kl) * sk] = T(dtr * (x - C(last_pos)));
+
+    // ---- backward substitution over k = nz-2 .. 0, pos PB levels ahead ----
+    T pcur[PB], pnxt[PB];
+#pragma unroll
+    for (int p = 0; p < PB; ++p) pcur[p] = pos[int64_t(max(kl - 1 - p, 0)) * sk];
+    int kt = kl - 1;
+    for (; kt - PB + 1 >= 0; kt -= PB) {
+#pragma unroll
+        for (int p = 0; p < PB; ++p) pnxt[p] = pos[int64_t(max(kt - PB - p, 0)) * sk];
+#pragma unroll
+        for (int p = 0; p < PB; ++p) {
             const int k = kt - p;
-            if (k < 0) break;
             const CD<C> cd = scratch[int64_t(k) * sstride];
-            x = (k == nz - 1) ? cd.d : cd.d - cd.c * x;
-            if (active) tstage[base + int64_t(k) * sk] = T(dtr * (x - pk[p]));
+            x = cd.d - cd.c * x;
+            if (active) tstage[int64_t(k) * sk] = T(dtr * (x - C(pcur[p])));
+        }
+#pragma unroll
+        for (int p = 0; p < PB; ++p) pcur[p] = pnxt[p];
+        publish(progress, 2 * nz - 1 - (kt - PB + 1));
+    }
+#pragma unroll
+    for (int p = 0; p < PB; ++p) {
+        const int k = kt - p;
+        if (k >= 0) {
+            const CD<C> cd = scratch[int64_t(k) * sstride];
+            x = cd.d - cd.c * x;
+            if (active) tstage[int64_t(k) * sk] = T(dtr * (x - C(pcur[p])));
         }
     }
 }
 
-template <typename T, typename C, bool LDS>
-__global__ __launch_bounds__(64) void vadv_kernel(Components comps, const T *__restrict__ wcon,
-                                                  CD<C> *__restrict__ gscratch, VadvArgs a) {
+// Helper wave: touches every 64-byte half line of the 64-column row segments
+// of stage, pos, tens, tensstage and wcon (one extra element for the wcon
+// neighbour / ragged tails), LEAD levels ahead of the compute wave, then the
+// pos segments backwards ahead of the backward substitution.  Addresses are
+// clamped into [0, nx] along i (the halo is at least 1 wide) and into
+// [0, nz-1] along k, so every touch is in bounds.
+template <typename T>
+__device__ float helper_wave(const Component &cp, const T *wcon, int64_t row_base, int64_t i0,
+                             const VadvArgs &a, int *progress) {
+    constexpr int EPT = 64 / int(sizeof(T));  // elements per touch
+    constexpr int NT = 64 / EPT + 1;          // touches per field row segment
+    constexpr int WIN = 32;                   // touch loads in flight
+    constexpr int64_t FPE = int64_t(sizeof(T)) / 4;  // floats per element
+    const int lane = threadIdx.x & 63;
+    const int nz = int(a.nz);
+    const int f = lane / NT, t = lane - (lane / NT) * NT;
+    const int64_t elem = min(i0 + int64_t(t) * EPT, a.nx);
+    const T *field = f == 0   ? static_cast<const T *>(cp.stage)
+                     : f == 1 ? static_cast<const T *>(cp.pos)
+                     : f == 2 ? static_cast<const T *>(cp.tens)
+                     : f == 3 ? static_cast<const T *>(cp.tensstage)
+                              : wcon;
+    const float *line = reinterpret_cast<const float *>(field + row_base + elem * a.si);
+    auto touch = [&](int k) { return line[int64_t(k) * a.sk * FPE]; };
+
+    float acc = 0.f;
+    float buf[WIN];
+#pragma unroll
+    for (int u = 0; u < WIN; ++u) buf[u] = touch(min(u, nz - 1));
+    for (int kb = WIN; kb < nz; kb += WIN) {
+        pace(progress, kb + WIN - LEAD);
+#pragma unroll
+        for (int u = 0; u < WIN; ++u) {
+            acc += buf[u];
+            buf[u] = touch(min(kb + u, nz - 1));
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < WIN; ++u) acc += buf[u];
+
+    // backward: pos row segments for k = nz-2 down to 0, NT lanes per level
+    constexpr int PER = 64 / NT;  // levels per touch instruction
+    const int64_t pelem = min(i0 + int64_t(lane % NT) * EPT, a.nx);
+    const float *pline = reinterpret_cast<const float *>(static_cast<const T *>(cp.pos) + row_base +
+                                                         pelem * a.si);
+    const int sub = lane / NT < PER ? lane / NT : 0;
+    for (int kt = nz - 2; kt >= 0; kt -= PER * WIN) {
+        pace(progress, 2 * nz - 1 - kt - LEAD);
+#pragma unroll
+        for (int u = 0; u < WIN; ++u) {
+            const int k = max(kt - u * PER - sub, 0);
+            buf[u] = pline[int64_t(k) * a.sk * FPE];
+        }
+#pragma unroll
+        for (int u = 0; u < WIN; ++u) acc += buf[u];
+    }
+    return acc;
+}
+
+template <typename T, typename C, bool LDS, int PF, bool NT, bool HELPER>
+__global__ __launch_bounds__(HELPER ? 128 : 64) __attribute__((amdgpu_waves_per_eu(1, LDS ? 2 : 8)))
+void vadv_kernel(Components comps, const T *__restrict__ wcon, CD<C> *__restrict__ gscratch,
+                 VadvArgs a) {
     extern __shared__ unsigned char lds_raw[];
-    const int lane = threadIdx.x;
+    __shared__ int s_progress;
+    const int lane = threadIdx.x & 63;
+    const int wave = HELPER ? int(threadIdx.x >> 6) : 0;
     const int64_t tile_i = blockIdx.x % a.tiles_i;
     const int64_t j = blockIdx.x / a.tiles_i;
-    const int64_t i = tile_i * 64 + lane;
+    const int64_t i0 = tile_i * 64;
+    const int64_t i = i0 + lane;
     const bool active = i < a.nx;
     const int64_t ic = active ? i : a.nx - 1;  // idle lanes re-read a valid column
     const int64_t base = ic * a.si + j * a.sj;
@@ -186,9 +321,290 @@ __global__ __launch_bounds__(64) void vadv_kernel(Components comps, const T *__r
         scratch = gscratch + int64_t(blockIdx.x) * 64 + lane;
         sstride = a.scratch_cols;
     }
-    for (int c = 0; c < a.ncomp; ++c)
-        solve_column<T, C, LDS>(comps.c[c], wcon, base, a.sk, int(a.nz), scratch, sstride,
-                                active);
+    for (int c = 0; c < a.ncomp; ++c) {
+        if constexpr (HELPER) {
+            if (threadIdx.x == 0) s_progress = -(1 << 30);
+            __syncthreads();
+            if (wave == 0)
+                solve_column<T, C, PF, NT>(comps.c[c], wcon, base, a.sk, int(a.nz), scratch,
+                                           sstride, active, &s_progress);
+            else {
+                const float sum = helper_wave<T>(comps.c[c], wcon, j * a.sj, i0, a, &s_progress);
+                // an empty asm that reads the sum keeps the touch loads alive
+                __asm__ volatile("" : : "v"(sum));
+            }
+            __syncthreads();
+        } else {
+            solve_column<T, C, PF, NT>(comps.c[c], wcon, base, a.sk, int(a.nz), scratch, sstride,
+                                       active, nullptr);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Variant 3: persistent pipelined column solver.
+//
+// A workgroup (one wave) processes a sequence of 64-column groups g_0, g_1,
+// ... (g_n = blockIdx + n * gridDim, so the groups in flight across the chip
+// are neighbours in memory).  The forward sweep of group n runs in the same
+// loop as the backward substitution of group n-1: at step s the backward
+// handles level nz-1-s of group n-1 and the forward level s-1 of group n.
+// Both use ONE LDS buffer of nz-1 (c', d') slots per lane: group n stores
+// level k at slot k when n is even and at slot nz-2-k when n is odd, so at
+// every step the backward reads exactly the slot the forward writes next
+// (read first, then write).  The backward's pos loads and tensstage stores
+// join the forward's loads in one prefetch pipeline, and the forward and
+// backward recurrences interleave.  Arithmetic is the same as above.
+// ---------------------------------------------------------------------------
+struct PipeArgs {
+    int64_t nx, ny, nz, si, sj, sk;
+    int32_t tiles_i;
+    int32_t ncomp;
+    int64_t ngroups;
+    uint32_t span_bytes;  // addressable bytes from each field's interior origin
+    uint32_t pad;
+};
+
+// Buffer-resource addressing: every field is one descriptor based at its
+// interior origin (out-of-range reads return 0, writes are dropped), a lane
+// adds its 32-bit column byte offset and a level adds the uniform byte
+// offset j*sj + k*sk, so a load is a single buffer instruction.
+constexpr int kRsrcWord3 = 0x00020000;
+
+template <typename T, int AUX>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, AUX));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, AUX));
+}
+
+template <typename T>
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, T v) {
+    if constexpr (sizeof(T) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(
+            __builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v), r,
+            voff, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, voff, soff, 0);
+}
+
+struct Rsrc {
+    __amdgpu_buffer_rsrc_t stage, pos, tens, ts, w;
+};
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), short(0), int(bytes), kRsrcWord3);
+}
+
+template <typename C>
+struct FwdState {
+    C st_m, st_c, w_c, wn_c, c_prev, d_prev;
+};
+
+// Column-group geometry: lane byte offsets and the group's row byte offset.
+struct GroupOff {
+    uint32_t lane, lane_wn, row;
+};
+
+template <typename T>
+__device__ __forceinline__ GroupOff group_offsets(const PipeArgs &a, int64_t g, int64_t wshift) {
+    const int64_t tile_i = g % a.tiles_i;
+    const int64_t j = g / a.tiles_i;
+    const int64_t i = tile_i * 64 + (threadIdx.x & 63);
+    // idle lanes past nx duplicate column nx-1: same inputs, same outputs,
+    // so their (identical) stores are harmless
+    const int64_t ic = i < a.nx ? i : a.nx - 1;
+    GroupOff o;
+    o.lane = uint32_t(ic * a.si * int64_t(sizeof(T)));
+    o.lane_wn = uint32_t((ic * a.si + wshift) * int64_t(sizeof(T)));
+    o.row = uint32_t(j * a.sj * int64_t(sizeof(T)));
+    return o;
+}
+
+// One group-iteration: forward of group `gf` (if FWD) fused with the
+// backward of group `gb` (if BWD).  `odd` is the slot orientation of the
+// forward group; the backward group has the opposite one.
+template <typename T, typename C, int PF, int AUX, bool FWD, bool BWD>
+__device__ __forceinline__ void pipe_iteration(const Rsrc &R, int64_t wshift, const PipeArgs &a,
+                                               int64_t gf, int64_t gb, bool odd, CD<C> *lds,
+                                               C &x, T &last_pos) {
+    const int nz = int(a.nz);
+    const int kl = nz - 1;
+    const uint32_t skb = uint32_t(a.sk * int64_t(sizeof(T)));
+    const C dtr = C(3.0 / 20.0);
+    const C bet_m = C(0.5), bet_p = C(0.5);
+    GroupOff F{}, B{};
+    if constexpr (FWD) F = group_offsets<T>(a, gf, wshift);
+    if constexpr (BWD) B = group_offsets<T>(a, gb, wshift);
+    auto fslot = [&](int k) { return odd ? (kl - 1 - k) : k; };
+    auto bslot = [&](int k) { return odd ? k : (kl - 1 - k); };
+
+    auto load_level = [&](int k, Level<T> &r) {  // k in [0, nz-2]
+        const uint32_t so = F.row + uint32_t(k) * skb;
+        r.pos = bload<T, 0>(R.pos, F.lane, so);
+        r.tens = bload<T, AUX>(R.tens, F.lane, so);
+        r.ts = bload<T, AUX>(R.ts, F.lane, so);
+        r.st = bload<T, AUX>(R.stage, F.lane, so + skb);
+        r.w = bload<T, AUX>(R.w, F.lane, so + skb);
+        r.wn = bload<T, AUX>(R.w, F.lane_wn, so + skb);
+    };
+    auto load_bpos = [&](int k) { return bload<T, 0>(R.pos, B.lane, B.row + uint32_t(k) * skb); };
+    auto store_out = [&](int k, C value) {
+        bstore<T>(R.ts, B.lane, B.row + uint32_t(k) * skb, T(value));
+    };
+
+    FwdState<C> f{};
+    // ---- step 0: backward level kl (register-resident) -------------------
+    if constexpr (BWD) store_out(kl, dtr * (x - C(last_pos)));
+    // ---- step 1: forward level 0, backward level kl-1 --------------------
+    Level<T> cur[PF], nxt[PF];
+    T bcur[PF], bnxt[PF];
+    {
+        Level<T> first{};
+        T st0 = T(0);
+        if constexpr (FWD) {
+            load_level(0, first);
+            st0 = bload<T, AUX>(R.stage, F.lane, F.row);
+#pragma unroll
+            for (int p = 0; p < PF; ++p) load_level(min(1 + p, kl - 1), cur[p]);
+        }
+        T bpos = T(0);
+        if constexpr (BWD) {
+            bpos = load_bpos(kl - 1);
+#pragma unroll
+            for (int p = 0; p < PF; ++p) bcur[p] = load_bpos(max(kl - 2 - p, 0));
+            const CD<C> cd = lds[int64_t(bslot(kl - 1)) * 64];
+            x = cd.d - cd.c * x;
+            store_out(kl - 1, dtr * (x - C(bpos)));
+        }
+        if constexpr (FWD) {
+            const C gcv = C(0.25) * (C(first.wn) + C(first.w));
+            const C cs = gcv * bet_m;
+            C cc = gcv * bet_p;
+            const C bcol = dtr - cc;
+            const C corr = -cs * (C(first.st) - C(st0));
+            C dd = ((dtr * C(first.pos) + C(first.tens)) + C(first.ts)) + corr;
+            cc = cc / bcol;
+            dd = dd / bcol;
+            lds[int64_t(fslot(0)) * 64] = CD<C>{cc, dd};
+            f = FwdState<C>{C(st0), C(first.st), C(first.w), C(first.wn), cc, dd};
+        }
+    }
+
+    // ---- steps 2 .. nz-1: forward level s-1 (middle), backward kl-s -----
+    auto step = [&](int s, const Level<T> &r, T bp) {
+        if constexpr (BWD) {
+            const int kb = kl - s;
+            const CD<C> cd = lds[int64_t(bslot(kb)) * 64];
+            x = cd.d - cd.c * x;
+            store_out(kb, dtr * (x - C(bp)));
+        }
+        if constexpr (FWD) {
+            const int k = s - 1;
+            const C gav = C(-0.25) * (f.wn_c + f.w_c);
+            const C w_n = C(r.w), wn_n = C(r.wn), st_n = C(r.st);
+            const C gcv = C(0.25) * (wn_n + w_n);
+            const C as = gav * bet_m;
+            const C cs = gcv * bet_m;
+            const C acol = gav * bet_p;
+            C cc = gcv * bet_p;
+            const C bcol = (dtr - acol) - cc;
+            const C corr = -as * (f.st_m - f.st_c) - cs * (st_n - f.st_c);
+            C dd = ((dtr * C(r.pos) + C(r.tens)) + C(r.ts)) + corr;
+            const C divided = C(1.0) / (bcol - f.c_prev * acol);
+            cc = cc * divided;
+            dd = (dd - f.d_prev * acol) * divided;
+            lds[int64_t(fslot(k)) * 64] = CD<C>{cc, dd};
+            f = FwdState<C>{f.st_c, st_n, w_n, wn_n, cc, dd};
+        }
+    };
+    auto prefetch = [&](int s0, Level<T> *lv, T *bp) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+            if constexpr (FWD) load_level(min(s0 + p - 1, kl - 1), lv[p]);
+            if constexpr (BWD) bp[p] = load_bpos(max(kl - (s0 + p), 0));
+        }
+    };
+    auto block = [&](int s0, const Level<T> *lv, const T *bp) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) step(s0 + p, lv[p], bp[p]);
+    };
+
+    // blocks alternate between the two register buffers (no copies)
+    int s = 2;
+    for (; s + 2 * PF - 1 <= kl; s += 2 * PF) {
+        prefetch(s + PF, nxt, bnxt);
+        block(s, cur, bcur);
+        prefetch(s + 2 * PF, cur, bcur);
+        block(s + PF, nxt, bnxt);
+    }
+    T lp = T(0), lt = T(0), lts = T(0);
+    if (s + PF - 1 <= kl) {
+        prefetch(s + PF, nxt, bnxt);
+        if constexpr (FWD) {
+            const uint32_t so = F.row + uint32_t(kl) * skb;
+            lp = bload<T, 0>(R.pos, F.lane, so);
+            lt = bload<T, AUX>(R.tens, F.lane, so);
+            lts = bload<T, AUX>(R.ts, F.lane, so);
+        }
+        block(s, cur, bcur);
+        s += PF;
+#pragma unroll
+        for (int p = 0; p < PF; ++p)
+            if (s + p <= kl) step(s + p, nxt[p], bnxt[p]);
+    } else {
+        if constexpr (FWD) {
+            const uint32_t so = F.row + uint32_t(kl) * skb;
+            lp = bload<T, 0>(R.pos, F.lane, so);
+            lt = bload<T, AUX>(R.tens, F.lane, so);
+            lts = bload<T, AUX>(R.ts, F.lane, so);
+        }
+#pragma unroll
+        for (int p = 0; p < PF; ++p)
+            if (s + p <= kl) step(s + p, cur[p], bcur[p]);
+    }
+
+    // ---- step nz: forward level kl (kept in registers) -------------------
+    if constexpr (FWD) {
+        const C gav = C(-0.25) * (f.wn_c + f.w_c);
+        const C as = gav * bet_m;
+        const C acol = gav * bet_p;
+        const C bcol = dtr - acol;
+        const C corr = -as * (f.st_m - f.st_c);
+        C dd = ((dtr * C(lp) + C(lt)) + C(lts)) + corr;
+        x = (dd - f.d_prev * acol) / (bcol - f.c_prev * acol);
+        last_pos = lp;
+    }
+}
+
+template <typename T, typename C, int PF, bool NT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void vadv_pipe_kernel(Components comps, const T *__restrict__ wcon, PipeArgs a) {
+    extern __shared__ unsigned char lds_raw[];
+    constexpr int AUX = NT ? 2 : 0;  // cache policy: non-temporal for streamed fields
+    CD<C> *lds = reinterpret_cast<CD<C> *>(lds_raw) + (threadIdx.x & 63);
+    const int64_t first = blockIdx.x;
+    if (first >= a.ngroups) return;
+    const int64_t count = (a.ngroups - first + gridDim.x - 1) / gridDim.x;
+    for (int c = 0; c < a.ncomp; ++c) {
+        const Component &cp = comps.c[c];
+        const Rsrc R{make_rsrc(cp.stage, a.span_bytes), make_rsrc(cp.pos, a.span_bytes),
+                     make_rsrc(cp.tens, a.span_bytes), make_rsrc(cp.tensstage, a.span_bytes),
+                     make_rsrc(wcon, a.span_bytes)};
+        C x = C(0);
+        T last_pos = T(0);
+        pipe_iteration<T, C, PF, AUX, true, false>(R, cp.wshift, a, first, 0, false, lds, x,
+                                                   last_pos);
+        for (int64_t n = 1; n < count; ++n)
+            pipe_iteration<T, C, PF, AUX, true, true>(R, cp.wshift, a, first + n * gridDim.x,
+                                                      first + (n - 1) * gridDim.x, (n & 1) != 0,
+                                                      lds, x, last_pos);
+        // the backward-only tail has one load per step: prefetch deeper
+        pipe_iteration<T, C, 32, AUX, false, true>(R, cp.wshift, a, 0,
+                                                   first + (count - 1) * gridDim.x,
+                                                   (count & 1) != 0, lds, x, last_pos);
+    }
 }
 
 int ensure_scratch(Context *ctx, void **scratch, size_t *have, size_t bytes) {
@@ -206,6 +622,64 @@ int ensure_scratch(Context *ctx, void **scratch, size_t *have, size_t bytes) {
 
 void *g_scratch[64] = {};
 size_t g_scratch_bytes[64] = {};
+
+template <int PF, bool NT, bool HELPER, typename T, typename C>
+hipError_t launch_kernel(bool lds, uint64_t blocks, size_t lds_bytes, const Components &comps,
+                         const T *wcon, CD<C> *gscratch, const VadvArgs &a, Context *ctx) {
+    if (lds) {
+        (void)hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&vadv_kernel<T, C, true, PF, NT, HELPER>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
+        hipLaunchKernelGGL((vadv_kernel<T, C, true, PF, NT, HELPER>), dim3(unsigned(blocks)),
+                           dim3(HELPER ? 128 : 64), lds_bytes, ctx->stream, comps, wcon, gscratch, a);
+    } else {
+        hipLaunchKernelGGL((vadv_kernel<T, C, false, PF, NT, false>), dim3(unsigned(blocks)),
+                           dim3(64), 0, ctx->stream, comps, wcon, gscratch, a);
+    }
+    return hipGetLastError();
+}
+
+template <int PF, typename T, typename C>
+hipError_t launch_depth(int variant, int flags, uint64_t blocks, size_t lds_bytes,
+                        const Components &comps, const T *wcon, CD<C> *gscratch, const VadvArgs &a,
+                        Context *ctx) {
+    const bool lds = variant == 1;
+    const bool nt = flags & 1, helper = lds && (flags & 2);
+    if (nt && helper)
+        return launch_kernel<PF, true, true>(lds, blocks, lds_bytes, comps, wcon, gscratch, a, ctx);
+    if (nt)
+        return launch_kernel<PF, true, false>(lds, blocks, lds_bytes, comps, wcon, gscratch, a, ctx);
+    if (helper)
+        return launch_kernel<PF, false, true>(lds, blocks, lds_bytes, comps, wcon, gscratch, a, ctx);
+    return launch_kernel<PF, false, false>(lds, blocks, lds_bytes, comps, wcon, gscratch, a, ctx);
+}
+
+// Static LDS of the variant-1 kernel (progress word), with slack for
+// alignment.
+constexpr size_t kStaticLds = 64;
+
+template <typename T, typename C, int PF, bool NT>
+hipError_t launch_pipe_k(unsigned grid, size_t lds_bytes, const Components &comps, const T *wcon,
+                         const PipeArgs &pa, Context *ctx) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&vadv_pipe_kernel<T, C, PF, NT>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
+    hipLaunchKernelGGL((vadv_pipe_kernel<T, C, PF, NT>), dim3(grid), dim3(64), lds_bytes,
+                       ctx->stream, comps, wcon, pa);
+    return hipGetLastError();
+}
+
+template <typename T, typename C>
+hipError_t launch_pipe(int depth, bool nt, unsigned grid, size_t lds_bytes, const Components &comps,
+                       const T *wcon, const PipeArgs &pa, Context *ctx) {
+    if (depth == 4)
+        return nt ? launch_pipe_k<T, C, 4, true>(grid, lds_bytes, comps, wcon, pa, ctx)
+                  : launch_pipe_k<T, C, 4, false>(grid, lds_bytes, comps, wcon, pa, ctx);
+    if (depth == 16)
+        return nt ? launch_pipe_k<T, C, 16, true>(grid, lds_bytes, comps, wcon, pa, ctx)
+                  : launch_pipe_k<T, C, 16, false>(grid, lds_bytes, comps, wcon, pa, ctx);
+    return nt ? launch_pipe_k<T, C, 8, true>(grid, lds_bytes, comps, wcon, pa, ctx)
+              : launch_pipe_k<T, C, 8, false>(grid, lds_bytes, comps, wcon, pa, ctx);
+}
 
 template <int P>
 int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_launch *l,
@@ -242,13 +716,18 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
     const uint64_t blocks = uint64_t(a.tiles_i) * uint64_t(g->ny);
     SB_REQUIRE(blocks < (uint64_t(1) << 26), "domain too large for one launch");
 
-    const size_t lds_bytes = size_t(g->nz) * 64 * sizeof(CD<C>);
+    const size_t lds_bytes = size_t(g->nz - 1) * 64 * sizeof(CD<C>);
+    const bool lds_fits = lds_bytes + kStaticLds <= ctx->lds_per_block;
     int variant = l ? l->variant : 0;
-    if (variant == 0) variant = lds_bytes <= ctx->lds_per_block ? 1 : 2;
-    SB_REQUIRE(variant == 1 || variant == 2, "vadv variant %d not available", variant);
-    SB_REQUIRE(variant != 1 || lds_bytes <= ctx->lds_per_block,
-               "nz = %lld needs %zu bytes of LDS per wave, device allows %zu",
-               (long long)g->nz, lds_bytes, ctx->lds_per_block);
+    const int64_t last_elem = (g->nx - 1 + g->hi) * g->si + (g->ny - 1 + g->hj) * g->sj +
+                              (g->nz - 1 + g->hk) * g->sk;
+    const bool buffer_ok = g->si > 0 && g->sj > 0 && g->sk > 0 &&
+                           uint64_t(last_elem + 1) * sizeof(T) < (uint64_t(1) << 32);
+    if (variant == 0) variant = lds_fits ? (buffer_ok ? 3 : 1) : 2;
+    SB_REQUIRE(variant >= 1 && variant <= 3, "vadv variant %d not available", variant);
+    SB_REQUIRE(variant == 2 || lds_fits,
+               "nz = %lld needs %zu bytes of LDS per workgroup, device allows %zu",
+               (long long)g->nz, lds_bytes + kStaticLds, ctx->lds_per_block);
 
     CD<C> *gscratch = nullptr;
     if (variant == 2) {
@@ -259,19 +738,43 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
             return 1;
         gscratch = static_cast<CD<C> *>(g_scratch[ctx->device & 63]);
     }
-    if (variant == 1) {
-        (void)hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&vadv_kernel<T, C, true>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
+    // defaults measured on MI355X: deep prefetch for tall columns, and
+    // non-temporal streamed loads when the fields are not much larger than
+    // the 256 MB Infinity Cache (they keep pos resident for the backward)
+    const double field_bytes = double(g->nx) * double(g->ny) * double(g->nz) * sizeof(T);
+    const int depth = (l && l->unroll) ? l->unroll : (g->nz >= 128 ? 8 : 4);
+    SB_REQUIRE(depth == 4 || depth == 8 || depth == 16, "vadv prefetch depth %d not in {4, 8, 16}",
+               depth);
+    // launch->bz: bit 0 non-temporal streamed loads, bit 1 helper wave (variant 1);
+    // 0 selects the default, -1 forces plain loads
+    int flags = l ? l->bz : 0;
+    if (flags == 0) flags = (6.0 * field_bytes < 512e6) ? 1 : 0;
+    if (flags < 0) flags = 0;
+    if (variant == 3) {
+        // byte span from a field's interior origin to its last addressable
+        // element (halo included); buffer offsets are 32-bit
+        const int64_t last = (g->nx - 1 + g->hi) * g->si + (g->ny - 1 + g->hj) * g->sj +
+                             (g->nz - 1 + g->hk) * g->sk;
+        const uint64_t span = uint64_t(last + 1) * sizeof(T);
+        SB_REQUIRE(g->si > 0 && g->sj > 0 && g->sk > 0 && span < (uint64_t(1) << 32),
+                   "pipelined vadv needs positive strides and fields below 4 GiB");
+        PipeArgs pa{a.nx, a.ny, a.nz, a.si, a.sj, a.sk, a.tiles_i, a.ncomp, int64_t(blocks),
+                    uint32_t(span), 0};
+        // persistent grid: launch->bx workgroups per CU (0: as many as LDS allows)
+        const int fit = int(size_t(ctx->lds_per_block) / (lds_bytes + kStaticLds));
+        const int per_cu = (l && l->bx > 0) ? min(l->bx, max(fit, 1)) : max(fit, 1);
+        const uint64_t resident = uint64_t(ctx->compute_units) * uint64_t(per_cu);
+        const unsigned grid = unsigned(blocks < resident ? blocks : resident);
+        return timed_launch(l, seconds, [&]() -> hipError_t {
+            return launch_pipe<T, C>(depth, flags & 1, grid, lds_bytes, comps, wcon, pa, ctx);
+        });
     }
     return timed_launch(l, seconds, [&]() -> hipError_t {
-        if (variant == 1)
-            hipLaunchKernelGGL((vadv_kernel<T, C, true>), dim3(unsigned(blocks)), dim3(64),
-                               lds_bytes, ctx->stream, comps, wcon, gscratch, a);
-        else
-            hipLaunchKernelGGL((vadv_kernel<T, C, false>), dim3(unsigned(blocks)), dim3(64), 0,
-                               ctx->stream, comps, wcon, gscratch, a);
-        return hipGetLastError();
+        switch (depth) {
+            case 4: return launch_depth<4>(variant, flags, blocks, lds_bytes, comps, wcon, gscratch, a, ctx);
+            case 16: return launch_depth<16>(variant, flags, blocks, lds_bytes, comps, wcon, gscratch, a, ctx);
+            default: return launch_depth<8>(variant, flags, blocks, lds_bytes, comps, wcon, gscratch, a, ctx);
+        }
     });
 }
 

@@ -118,14 +118,15 @@ HDIFF_DOMAINS = [(70, 37, 9), (64, 16, 1), (1, 1, 1), (129, 33, 3), (5, 70, 2)]
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
-@pytest.mark.parametrize("variant,tile", [(1, 8), (1, 16), (1, 32), (3, 0)])
+@pytest.mark.parametrize("variant,tile,planes", [(1, 8, 1), (1, 16, 8), (1, 32, 2), (1, 16, 16),
+                                                  (3, 0, 0)])
 @pytest.mark.parametrize("domain", HDIFF_DOMAINS)
-def test_hdiff(gpu, prec, variant, tile, domain):
+def test_hdiff(gpu, prec, variant, tile, planes, domain):
     halo = (2, 3, 0)
     dtype = _dtype(prec)
     inp, coeff = make_fields(domain, halo, dtype, count=2, seed=sum(domain))
     out = _with_sentinel(make_fields(domain, halo, dtype)[0])
-    opts = gpu.launch_options(variant=variant, block=(64, tile, 1))
+    opts = gpu.launch_options(variant=variant, block=(64, tile, planes))
     got, = _run(gpu, "sb_hdiff", [inp, coeff, out], [2], domain, halo, _prec(gpu, prec),
                 launch=opts)
     expected = _expected(lambda a, b: oracle.hdiff(a, b, domain, halo), [inp, coeff], prec)
@@ -158,7 +159,7 @@ def _vadv_expected(fields, domain, halo, prec, components):
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
-@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("variant", [1, 2, 3])
 @pytest.mark.parametrize("all_components", [False, True])
 @pytest.mark.parametrize("domain", VADV_DOMAINS)
 def test_vadv(gpu, prec, variant, all_components, domain):
@@ -176,7 +177,32 @@ def test_vadv(gpu, prec, variant, all_components, domain):
         assert np.array_equal(g[inner(domain, halo)], e)
 
 
-def _run_vadv(native, fields, written, domain, halo, code, all_components, variant):
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("domain", [(128, 700, 19), (70, 530, 9), (64, 1300, 2)])
+@pytest.mark.parametrize("depth", [4, 8])
+def test_vadv_pipelined_many_groups(gpu, prec, domain, depth):
+    # more column groups than resident workgroups: every workgroup streams
+    # several groups through the fused forward/backward pipeline
+    halo = (1, 0, 0)
+    dtype = _dtype(prec)
+    fields = make_fields(domain, halo, dtype, count=8, seed=sum(domain))
+    got, = _run_vadv(gpu, fields, [3], domain, halo, _prec(gpu, prec), False, 3,
+                     extra=dict(unroll=depth, bz=1))
+    expected = _expected(lambda *f: oracle.vadv(*f, domain, halo), fields[:5], prec)
+    assert np.array_equal(got[inner(domain, halo)], expected)
+
+
+@pytest.mark.parametrize("flags", [1, 2, 3])
+@pytest.mark.parametrize("variant,depth", [(1, 4), (1, 16), (2, 8)])
+def test_vadv_streaming_options(gpu, flags, variant, depth):
+    domain, halo = (130, 5, 41), (1, 1, 0)
+    fields = make_fields(domain, halo, np.float64, count=8, seed=11)
+    got, = _run_vadv(gpu, fields, [3], domain, halo, gpu.SB_F64, False, variant,
+                     extra=dict(bz=flags, unroll=depth))
+    assert np.array_equal(got[inner(domain, halo)], oracle.vadv(*fields[:5], domain, halo))
+
+
+def _run_vadv(native, fields, written, domain, halo, code, all_components, variant, extra=None):
     from stencil_benchmarks_amd.device import DeviceField, geometry
 
     devs = [DeviceField(f, halo) for f in fields]
@@ -184,8 +210,10 @@ def _run_vadv(native, fields, written, domain, halo, code, all_components, varia
         d.upload(f)
     geom = geometry(domain, [s // fields[0].itemsize for s in fields[0].strides], halo)
     array = (ctypes.c_void_p * len(devs))(*[d.pointer() for d in devs])
-    native.call("sb_vadv", code, int(all_components), array, geom,
-                native.launch_options(variant=variant), None)
+    opts = native.launch_options(variant=variant)
+    for key, value in (extra or {}).items():
+        setattr(opts, key, value)
+    native.call("sb_vadv", code, int(all_components), array, geom, opts, None)
     native.call("sb_synchronize")
     results = []
     for w in written:
