@@ -168,6 +168,44 @@ int sb_stream_kernel(int prec, int op, void *a, void *b, void *c, size_t n,
 int sb_stream_check(int prec, const void *x, size_t n, double expected,
                     double *max_rel_err);
 
+/* ---- batched subdomains ---------------------------------------------------
+ * No counterpart in the reference (its stencils run one domain per launch).
+ * A subdomain owns seven fields with one shared geometry: the state pair
+ * u[0] / u[1] (hdiff input and output, swapped every step), coeff, and the
+ * vadv fields upos, utens, utensstage (read-write) and wcon.  One timestep is
+ *     u[1-parity] = hdiff(u[parity], coeff)        (interior only)
+ *     utensstage  = vadv(ustage = u[1-parity], upos, utens, utensstage, wcon)
+ * (u component).  The halos of u[0] and u[1] must hold the same boundary
+ * values; they are never written.  Requirements: hi, hj >= 2, nz >= 2,
+ * positive strides, every field below 4 GiB, nz small enough that a 64-column
+ * Thomas buffer fits in LDS (nz <= 160 with FP64 arithmetic).
+ * Launch options: hdiff_launch->by / ->bz / ->unroll = tile height / planes
+ * per item / non-temporal flags (as sb_hdiff); vadv_launch->unroll = prefetch
+ * depth (4 or 8), ->bz = 1 for non-temporal streamed loads. */
+typedef struct sb_subdomain {
+    void *u[2];
+    void *coeff;
+    void *upos, *utens, *utensstage, *wcon;
+    sb_geom geom;
+    int32_t parity; /* index of the current state in u[] */
+    int32_t reserved;
+} sb_subdomain;
+
+/* One timestep for `count` subdomains (two persistent launches drawing
+ * tiles and column groups from a device work queue).  Does not flip the
+ * parities; the caller does. */
+int sb_batch_step(int prec, const sb_subdomain *subs, int count, const sb_launch *hdiff_launch,
+                  const sb_launch *vadv_launch, double *seconds);
+/* A fixed set of subdomains advanced `steps` steps per run: device tables
+ * are built once; with use_graph != 0 the whole step sequence is captured
+ * once in a hipGraph and replayed.  After a run subdomain i holds its state
+ * in u[(parity_i + steps) % 2]. */
+int sb_batch_plan_create(int prec, const sb_subdomain *subs, int count, int steps,
+                         const sb_launch *hdiff_launch, const sb_launch *vadv_launch,
+                         void **plan);
+int sb_batch_plan_run(void *plan, int use_graph, double *seconds);
+int sb_batch_plan_destroy(void *plan);
+
 #ifdef __cplusplus
 }
 #endif

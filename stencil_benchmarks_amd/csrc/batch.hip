@@ -1,0 +1,443 @@
+// Batched timesteps over many subdomains of uneven size.
+//
+// One timestep of a subdomain (the wiring of the batched and served
+// workloads; the reference has no counterpart):
+//   u_next     = hdiff(u, coeff)                       (interior of u_next)
+//   utensstage = vadv(ustage = u_next, upos, utens, utensstage, wcon)
+// and the caller swaps u and u_next (parity).  The halo of u / u_next is
+// never written: it is a fixed boundary, identical in both buffers.
+//
+// Each step is two persistent launches (grid = resident workgroups) that
+// draw work from a device queue: an atomic counter hands out global item
+// indices, and a workgroup maps an index to (subdomain, local item) by a
+// binary search over per-subdomain prefix sums.  Subdomains are ordered by
+// cost, largest first, so the tail of the queue holds the small items.
+//   hdiff items: (subdomain, 64 x TY tile, KC-plane chunk), the tile worker
+//     of the plain kernel (hdiff_tile.h);
+//   vadv items:  a run of up to CHUNK consecutive 64-column groups of one
+//     subdomain, streamed through the fused forward/backward pipeline of
+//     the plain kernel (vadv_column.h) with an LDS buffer sized for the
+//     tallest column in the batch.
+// The counters are cleared by a small kernel at the start of every
+// sequence (no memset nodes, so the sequence can be captured in a
+// hipGraph and replayed).  Every workgroup exits once the counter passes
+// the item count.
+#include <algorithm>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "hdiff_tile.h"
+#include "vadv_column.h"
+
+namespace sb {
+namespace {
+
+constexpr int CHUNK = 32;  // column groups per vadv work item
+
+// Device descriptor of one subdomain for one step parity.
+struct SubDev {
+    const void *inp;   // u (hdiff input)
+    void *out;         // u_next (hdiff output, vadv stage)
+    const void *coeff;
+    const void *pos, *tens;
+    void *ts;
+    const void *wcon;
+    int64_t nx, ny, nz, si, sj, sk;
+    int32_t tiles_i, tiles_j;
+    uint32_t span_bytes;
+    int32_t pad;
+};
+
+__device__ __forceinline__ int find_sub(const int64_t *prefix, int count, int64_t item) {
+    // largest s with prefix[s] <= item (prefix[0] = 0, prefix[count] = total)
+    int lo = 0, hi = count;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= item)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void reset_counters(unsigned long long *counters, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counters[i] = 0ull;
+}
+
+template <typename T, typename C, int TY, int RPT, int KC, int FLAGS>
+__global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
+    const SubDev *__restrict__ subs, const int64_t *__restrict__ prefix, int count,
+    unsigned long long *__restrict__ counter) {
+    using Tile = HdiffTile<T, C, TY, RPT, FLAGS>;
+    __shared__ C s_in[Tile::IN_ELEMS];
+    __shared__ C s_lap[Tile::LAP_ELEMS];
+    __shared__ long long s_item;
+    const int64_t total = prefix[count];
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    for (;;) {
+        if (tid == 0) s_item = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const int64_t item = s_item;
+        __syncthreads();
+        if (item >= total) break;
+        const int s = find_sub(prefix, count, item);
+        const SubDev d = subs[s];
+        const int64_t local = item - prefix[s];
+        const int64_t tiles = int64_t(d.tiles_i) * d.tiles_j;
+        const int64_t kc = local / tiles;
+        const int64_t t = local - kc * tiles;
+        const int64_t i0 = (t % d.tiles_i) * Tile::TX;
+        const int64_t j0 = (t / d.tiles_i) * TY;
+        const int64_t k_begin = kc * KC;
+        const int64_t k_end = min(k_begin + int64_t(KC), d.nz);
+        const HdiffGeom g{d.nx, d.ny, d.nz, d.si, d.sj, d.sk};
+        Tile::run(static_cast<const T *>(d.inp), static_cast<const T *>(d.coeff),
+                  static_cast<T *>(d.out), g, i0, j0, k_begin, k_end, s_in, s_lap);
+    }
+}
+
+template <typename T, typename C, int PF, bool NT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void vadv_batch_kernel(const SubDev *__restrict__ subs, const int64_t *__restrict__ prefix,
+                       int count, unsigned long long *__restrict__ counter) {
+    extern __shared__ unsigned char lds_raw[];
+    constexpr int AUX = NT ? 2 : 0;
+    CD<C> *lds = reinterpret_cast<CD<C> *>(lds_raw) + (threadIdx.x & 63);
+    const int64_t total = prefix[count];
+    for (;;) {
+        unsigned long long ticket = 0;
+        if (threadIdx.x == 0) ticket = atomicAdd(counter, 1ull);
+        const int64_t item = int64_t(__shfl(ticket, 0));
+        if (item >= total) break;
+        const int s = find_sub(prefix, count, item);
+        const SubDev d = subs[s];
+        const int64_t groups = int64_t(d.tiles_i) * d.ny;
+        const int64_t g0 = (item - prefix[s]) * CHUNK;
+        const int64_t n = min(int64_t(CHUNK), groups - g0);
+        const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, d.tiles_i, 1, groups,
+                         d.span_bytes, 0};
+        const Rsrc R{make_rsrc(d.out, d.span_bytes), make_rsrc(d.pos, d.span_bytes),
+                     make_rsrc(d.tens, d.span_bytes), make_rsrc(d.ts, d.span_bytes),
+                     make_rsrc(d.wcon, d.span_bytes)};
+        const int64_t wshift = d.si;  // u component: wcon neighbour at i+1
+        C x = C(0);
+        T last_pos = T(0);
+        pipe_iteration<T, C, PF, AUX, true, false>(R, wshift, a, g0, 0, false, lds, x, last_pos);
+        for (int64_t q = 1; q < n; ++q)
+            pipe_iteration<T, C, PF, AUX, true, true>(R, wshift, a, g0 + q, g0 + q - 1,
+                                                      (q & 1) != 0, lds, x, last_pos);
+        pipe_iteration<T, C, 32, AUX, false, true>(R, wshift, a, 0, g0 + n - 1, (n & 1) != 0, lds,
+                                                   x, last_pos);
+    }
+}
+
+// ---- host side ---------------------------------------------------------------
+
+struct Plan {
+    int prec = 0;
+    int count = 0;
+    int steps = 0;
+    int max_nz = 0;
+    sb_launch hl{}, vl{};
+    // device tables: SubDev arrays for flip 0 / 1, prefix sums, counters
+    void *dev = nullptr;
+    size_t dev_bytes = 0;
+    SubDev *subs[2] = {nullptr, nullptr};
+    int64_t *hprefix = nullptr, *vprefix = nullptr;
+    unsigned long long *counters = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    // host copies
+    std::vector<SubDev> host_subs[2];
+    std::vector<int64_t> host_hprefix, host_vprefix;
+};
+
+int hdiff_ty(const sb_launch *l) { return (l && l->by) ? l->by : 16; }
+int hdiff_kc(const sb_launch *l, int prec) {
+    return (l && l->bz) ? l->bz : (prec == SB_F64 ? 2 : 8);
+}
+int hdiff_flags(const sb_launch *l) { return l && l->unroll ? (l->unroll & 3) : 2; }
+int vadv_pf(const sb_launch *l) { return (l && l->unroll) ? l->unroll : 8; }
+int vadv_nt(const sb_launch *l) { return (l && l->bz) ? (l->bz > 0 ? 1 : 0) : 0; }
+
+int validate(int prec, const sb_subdomain *subs, int count) {
+    SB_REQUIRE(prec == SB_F64 || prec == SB_F32 || prec == SB_F32_F64, "invalid precision %d", prec);
+    SB_REQUIRE(subs || count == 0, "NULL subdomain array");
+    SB_REQUIRE(count >= 0, "negative subdomain count");
+    const size_t esize = prec == SB_F64 ? 8 : 4;
+    for (int i = 0; i < count; ++i) {
+        const sb_subdomain &s = subs[i];
+        const sb_geom &g = s.geom;
+        SB_REQUIRE(s.u[0] && s.u[1] && s.coeff && s.upos && s.utens && s.utensstage && s.wcon,
+                   "subdomain %d: NULL field pointer", i);
+        SB_REQUIRE(g.nx > 0 && g.ny > 0 && g.nz >= 2, "subdomain %d: empty domain or nz < 2", i);
+        SB_REQUIRE(g.hi >= 2 && g.hj >= 2, "subdomain %d: horizontal halo must be >= 2", i);
+        SB_REQUIRE(g.si > 0 && g.sj > 0 && g.sk > 0, "subdomain %d: strides must be positive", i);
+        const int64_t last = (g.nx - 1 + g.hi) * g.si + (g.ny - 1 + g.hj) * g.sj +
+                             (g.nz - 1 + g.hk) * g.sk;
+        SB_REQUIRE(uint64_t(last + 1) * esize < (uint64_t(1) << 32),
+                   "subdomain %d: fields must be below 4 GiB", i);
+        SB_REQUIRE(s.parity == 0 || s.parity == 1, "subdomain %d: parity must be 0 or 1", i);
+    }
+    return 0;
+}
+
+// Build descriptors and prefix sums (largest subdomains first).
+void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch *hl, Plan &p) {
+    const size_t esize = prec == SB_F64 ? 8 : 4;
+    const int ty = hdiff_ty(hl), kc = hdiff_kc(hl, prec);
+    std::vector<int> order(count);
+    std::iota(order.begin(), order.end(), 0);
+    auto cost = [&](int i) {
+        const sb_geom &g = subs[i].geom;
+        return double(g.nx) * double(g.ny) * double(g.nz);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
+    p.max_nz = 2;
+    for (int flip = 0; flip < 2; ++flip) p.host_subs[flip].resize(count);
+    p.host_hprefix.assign(count + 1, 0);
+    p.host_vprefix.assign(count + 1, 0);
+    for (int r = 0; r < count; ++r) {
+        const sb_subdomain &s = subs[order[r]];
+        const sb_geom &g = s.geom;
+        const int64_t last = (g.nx - 1 + g.hi) * g.si + (g.ny - 1 + g.hj) * g.sj +
+                             (g.nz - 1 + g.hk) * g.sk;
+        for (int flip = 0; flip < 2; ++flip) {
+            const int cur = (s.parity + flip) & 1;
+            SubDev d{};
+            d.inp = s.u[cur];
+            d.out = s.u[cur ^ 1];
+            d.coeff = s.coeff;
+            d.pos = s.upos;
+            d.tens = s.utens;
+            d.ts = s.utensstage;
+            d.wcon = s.wcon;
+            d.nx = g.nx;
+            d.ny = g.ny;
+            d.nz = g.nz;
+            d.si = g.si;
+            d.sj = g.sj;
+            d.sk = g.sk;
+            d.tiles_i = int32_t((g.nx + 63) / 64);
+            d.tiles_j = int32_t((g.ny + ty - 1) / ty);
+            d.span_bytes = uint32_t(uint64_t(last + 1) * esize);
+            p.host_subs[flip][r] = d;
+        }
+        const int64_t tiles_i = (g.nx + 63) / 64;
+        const int64_t hitems = tiles_i * ((g.ny + ty - 1) / ty) * ((g.nz + kc - 1) / kc);
+        const int64_t vitems = (tiles_i * g.ny + CHUNK - 1) / CHUNK;
+        p.host_hprefix[r + 1] = p.host_hprefix[r] + hitems;
+        p.host_vprefix[r + 1] = p.host_vprefix[r] + vitems;
+        p.max_nz = std::max<int>(p.max_nz, int(g.nz));
+    }
+}
+
+template <typename T, typename C>
+size_t vadv_lds_bytes(int max_nz) {
+    return size_t(max_nz - 1) * 64 * sizeof(CD<C>);
+}
+
+// Enqueue one step (both kernels) on ctx->stream.
+template <typename T, typename C>
+hipError_t enqueue_step(const Plan &p, int flip, unsigned long long *counters, Context *ctx) {
+    const int ty = hdiff_ty(&p.hl), kc = hdiff_kc(&p.hl, p.prec), fl = hdiff_flags(&p.hl);
+    const int pf = vadv_pf(&p.vl);
+    const bool nt = vadv_nt(&p.vl);
+    const SubDev *subs = p.subs[flip];
+    // hdiff: resident grid
+    int per_cu = 0;
+    auto launch_h = [&](auto kernel, int nthreads) -> hipError_t {
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(kernel), nthreads, 0);
+        if (e != hipSuccess) return e;
+        const unsigned grid = unsigned(std::max(1, per_cu) * ctx->compute_units);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(64, nthreads / 64), 0, ctx->stream, subs,
+                           p.hprefix, p.count, counters);
+        return hipGetLastError();
+    };
+#define SB_HB(TY_, RPT_, KC_, FL_)                                                          \
+    if (ty == TY_ && kc == KC_ && fl == FL_)                                                \
+        return_h = launch_h(hdiff_batch_kernel<T, C, TY_, RPT_, KC_, FL_>, 64 * (TY_ / RPT_));
+    hipError_t return_h = hipErrorInvalidValue;
+    SB_HB(16, 4, 2, 2) else SB_HB(16, 4, 2, 0) else SB_HB(16, 4, 8, 2) else SB_HB(16, 4, 8, 0)
+        else SB_HB(8, 2, 1, 2) else SB_HB(32, 8, 2, 2) else SB_HB(16, 4, 4, 2) else SB_HB(16, 4, 1, 2)
+#undef SB_HB
+    if (return_h != hipSuccess) return return_h;
+    // vadv
+    const size_t lds = vadv_lds_bytes<T, C>(p.max_nz);
+    const int fit = std::max(1, int(ctx->lds_per_block / lds));
+    const unsigned vgrid = unsigned(fit * ctx->compute_units);
+    auto launch_v = [&](auto kernel) -> hipError_t {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        hipLaunchKernelGGL(kernel, dim3(vgrid), dim3(64), lds, ctx->stream, subs, p.vprefix,
+                           p.count, counters + 1);
+        return hipGetLastError();
+    };
+    if (pf == 4) return nt ? launch_v(vadv_batch_kernel<T, C, 4, true>) : launch_v(vadv_batch_kernel<T, C, 4, false>);
+    return nt ? launch_v(vadv_batch_kernel<T, C, 8, true>) : launch_v(vadv_batch_kernel<T, C, 8, false>);
+}
+
+template <typename T, typename C>
+hipError_t enqueue_sequence(const Plan &p, Context *ctx) {
+    hipLaunchKernelGGL(reset_counters, dim3((2 * p.steps + 255) / 256), dim3(256), 0, ctx->stream,
+                       p.counters, 2 * p.steps);
+    hipError_t e = hipGetLastError();
+    for (int s = 0; s < p.steps && e == hipSuccess; ++s)
+        e = enqueue_step<T, C>(p, s & 1, p.counters + 2 * s, ctx);
+    return e;
+}
+
+hipError_t enqueue_any(const Plan &p, Context *ctx) {
+    if (p.prec == SB_F64) return enqueue_sequence<double, double>(p, ctx);
+    if (p.prec == SB_F32) return enqueue_sequence<float, float>(p, ctx);
+    return enqueue_sequence<float, double>(p, ctx);
+}
+
+int upload_tables(Plan &p, Context *ctx) {
+    const size_t subs_bytes = sizeof(SubDev) * std::max(1, p.count);
+    const size_t pre_bytes = sizeof(int64_t) * (p.count + 1);
+    const size_t ctr_bytes = sizeof(unsigned long long) * 2 * std::max(1, p.steps);
+    const size_t need = 2 * subs_bytes + 2 * pre_bytes + ctr_bytes + 256;
+    if (p.dev_bytes < need) {
+        if (p.dev) SB_HIP(hipFree(p.dev));
+        p.dev = nullptr;
+        SB_HIP(hipMalloc(&p.dev, need));
+        p.dev_bytes = need;
+    }
+    char *base = static_cast<char *>(p.dev);
+    p.subs[0] = reinterpret_cast<SubDev *>(base);
+    p.subs[1] = reinterpret_cast<SubDev *>(base + subs_bytes);
+    p.hprefix = reinterpret_cast<int64_t *>(base + 2 * subs_bytes);
+    p.vprefix = reinterpret_cast<int64_t *>(base + 2 * subs_bytes + pre_bytes);
+    p.counters = reinterpret_cast<unsigned long long *>(base + 2 * subs_bytes + 2 * pre_bytes);
+    if (p.count) {
+        SB_HIP(hipMemcpyAsync(p.subs[0], p.host_subs[0].data(), sizeof(SubDev) * p.count,
+                              hipMemcpyHostToDevice, ctx->stream));
+        SB_HIP(hipMemcpyAsync(p.subs[1], p.host_subs[1].data(), sizeof(SubDev) * p.count,
+                              hipMemcpyHostToDevice, ctx->stream));
+    }
+    SB_HIP(hipMemcpyAsync(p.hprefix, p.host_hprefix.data(), pre_bytes, hipMemcpyHostToDevice,
+                          ctx->stream));
+    SB_HIP(hipMemcpyAsync(p.vprefix, p.host_vprefix.data(), pre_bytes, hipMemcpyHostToDevice,
+                          ctx->stream));
+    SB_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int check_lds(const Plan &p, Context *ctx) {
+    const size_t lds = p.prec == SB_F32 ? vadv_lds_bytes<float, float>(p.max_nz)
+                                        : vadv_lds_bytes<double, double>(p.max_nz);
+    SB_REQUIRE(lds <= ctx->lds_per_block,
+               "tallest column nz = %d needs %zu bytes of LDS, device allows %zu", p.max_nz, lds,
+               ctx->lds_per_block);
+    return 0;
+}
+
+int run_plan(Plan &p, int use_graph, double *seconds) {
+    Context *ctx = context();
+    if (!ctx) return 1;
+    if (p.count == 0 || p.steps == 0) {
+        if (seconds) *seconds = 0.0;
+        return 0;
+    }
+    if (use_graph && !p.exec) {
+        SB_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const hipError_t e = enqueue_any(p, ctx);
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+        SB_HIP(e);
+        SB_HIP(e2);
+        p.graph = graph;
+        SB_HIP(hipGraphInstantiate(&p.exec, p.graph, nullptr, nullptr, 0));
+    }
+    if (seconds) SB_HIP(hipEventRecord(ctx->start, ctx->stream));
+    if (use_graph)
+        SB_HIP(hipGraphLaunch(p.exec, ctx->stream));
+    else
+        SB_HIP(enqueue_any(p, ctx));
+    if (seconds) {
+        SB_HIP(hipEventRecord(ctx->stop, ctx->stream));
+        SB_HIP(hipEventSynchronize(ctx->stop));
+        float ms = 0.f;
+        SB_HIP(hipEventElapsedTime(&ms, ctx->start, ctx->stop));
+        *seconds = double(ms) * 1e-3;
+    }
+    return 0;
+}
+
+void destroy_plan(Plan *p) {
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->dev) (void)hipFree(p->dev);
+    delete p;
+}
+
+}  // namespace
+}  // namespace sb
+
+using namespace sb;
+
+extern "C" {
+
+int sb_batch_plan_create(int prec, const sb_subdomain *subs, int count, int steps,
+                         const sb_launch *hdiff_launch, const sb_launch *vadv_launch,
+                         void **plan) {
+    SB_REQUIRE(plan, "NULL plan pointer");
+    SB_REQUIRE(steps >= 0, "negative step count");
+    if (validate(prec, subs, count)) return 2;
+    Context *ctx = context();
+    if (!ctx) return 1;
+    std::unique_ptr<Plan> p(new Plan());
+    p->prec = prec;
+    p->count = count;
+    p->steps = steps;
+    if (hdiff_launch) p->hl = *hdiff_launch;
+    if (vadv_launch) p->vl = *vadv_launch;
+    build_tables(prec, subs, count, hdiff_launch, *p);
+    if (check_lds(*p, ctx)) return 2;
+    if (upload_tables(*p, ctx)) {
+        destroy_plan(p.release());
+        return 1;
+    }
+    *plan = p.release();
+    return 0;
+}
+
+int sb_batch_plan_run(void *plan, int use_graph, double *seconds) {
+    SB_REQUIRE(plan, "NULL plan");
+    return run_plan(*static_cast<Plan *>(plan), use_graph, seconds);
+}
+
+int sb_batch_plan_destroy(void *plan) {
+    Context *ctx = context();
+    if (ctx) SB_HIP(hipStreamSynchronize(ctx->stream));
+    destroy_plan(static_cast<Plan *>(plan));
+    return 0;
+}
+
+int sb_batch_step(int prec, const sb_subdomain *subs, int count, const sb_launch *hdiff_launch,
+                  const sb_launch *vadv_launch, double *seconds) {
+    if (validate(prec, subs, count)) return 2;
+    Context *ctx = context();
+    if (!ctx) return 1;
+    // one reusable single-step plan per device and thread
+    thread_local Plan *scratch_plan[64] = {};
+    Plan *&p = scratch_plan[ctx->device & 63];
+    if (!p) p = new Plan();
+    p->prec = prec;
+    p->count = count;
+    p->steps = 1;
+    p->hl = hdiff_launch ? *hdiff_launch : sb_launch{};
+    p->vl = vadv_launch ? *vadv_launch : sb_launch{};
+    build_tables(prec, subs, count, hdiff_launch, *p);
+    if (check_lds(*p, ctx)) return 2;
+    if (upload_tables(*p, ctx)) return 1;
+    return run_plan(*p, 0, seconds);
+}
+
+}  // extern "C"

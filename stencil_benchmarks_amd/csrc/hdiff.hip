@@ -23,7 +23,7 @@
 // (stencils/cuda_hip/horizontal_diffusion.py:50-96) with the oracle's
 // arithmetic order.  Kept as an independent implementation to cross-check
 // variant 1.
-#include "sb_internal.h"
+#include "hdiff_tile.h"
 
 namespace sb {
 namespace {
@@ -35,10 +35,6 @@ struct HdiffArgs {
     int32_t pad;
 };
 
-template <typename C>
-__device__ __forceinline__ C limited(C flux, C gradient) {
-    return flux * gradient > C(0) ? C(0) : flux;
-}
 
 // Variant 1 kernel.  Staging: each of the NW waves loads rows w, w+NW, ...
 // of the tile's 64 main columns (one coalesced 64-element row segment per
@@ -49,140 +45,26 @@ __device__ __forceinline__ C limited(C flux, C gradient) {
 // tile: the loads for plane k+1 are issued into registers before plane k
 // is computed and written to the other half of a double-buffered LDS tile
 // afterwards, so each wave always has a plane's worth of loads in flight.
-template <typename T, typename C, int TY, int RPT, int KC>
+
+
+// FLAGS bit 0: non-temporal stores of out; bit 1: non-temporal loads of coeff
+// (neither is re-read; inp is, by the neighbouring tiles' halos).
+template <typename T, typename C, int TY, int RPT, int KC, int FLAGS>
 __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_lds_kernel(
     const T *__restrict__ inp, const T *__restrict__ coeff, T *__restrict__ out, HdiffArgs a) {
-    constexpr int TX = 64;
-    constexpr int NW = TY / RPT;          // waves per workgroup
-    constexpr int LW = TX + 4;            // staged columns: tile i = -2 .. TX+1
-    constexpr int LH = TY + 4;            // staged rows:    tile j = -2 .. TY+1
-    constexpr int PW = TX + 2;            // published Laplacian columns: -1 .. TX
-    constexpr int MAIN = (LH + NW - 1) / NW;
-    __shared__ C s_in[2][LH * LW];
-    __shared__ C s_lap[TY * PW];
-
+    using Tile = HdiffTile<T, C, TY, RPT, FLAGS>;
+    __shared__ C s_in[Tile::IN_ELEMS];
+    __shared__ C s_lap[Tile::LAP_ELEMS];
     const uint32_t tiles = uint32_t(a.tiles_i) * uint32_t(a.tiles_j);
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
     const uint32_t kc = b / tiles;
     const uint32_t t = b - kc * tiles;
-    const int64_t i0 = int64_t(t % a.tiles_i) * TX;
+    const int64_t i0 = int64_t(t % a.tiles_i) * Tile::TX;
     const int64_t j0 = int64_t(t / a.tiles_i) * TY;
     const int64_t k_begin = int64_t(kc) * KC;
     const int64_t k_end = min(k_begin + int64_t(KC), a.nz);
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int tid = ty * TX + tx;
-
-    // ---- per-thread staging offsets (plane-independent) -----------------
-    int64_t off_main[MAIN];
-    int lds_main[MAIN];
-#pragma unroll
-    for (int m = 0; m < MAIN; ++m) {
-        const int r = ty + NW * m;
-        const int rr = r < LH ? r : LH - 1;
-        const int64_t gi = min(i0 + tx, a.nx + 1);
-        const int64_t gj = min(j0 + rr - 2, a.ny + 1);
-        off_main[m] = gi * a.si + gj * a.sj;
-        lds_main[m] = r < LH ? rr * LW + tx + 2 : -1;
-    }
-    const bool has_halo = tid < 4 * LH;
-    int64_t off_halo = 0;
-    int lds_halo = -1;
-    if (has_halo) {
-        const int r = tid >> 2, q = tid & 3;
-        const int c = q < 2 ? q : TX + q;  // 0, 1, TX+2, TX+3
-        const int64_t gi = min(i0 + c - 2, a.nx + 1);
-        const int64_t gj = min(j0 + r - 2, a.ny + 1);
-        off_halo = gi * a.si + gj * a.sj;
-        lds_halo = r * LW + c;
-    }
-    const int64_t gi_out = i0 + tx;
-    int64_t off_coeff[RPT];
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int64_t gi = min(gi_out, a.nx - 1);
-        const int64_t gj = min(j0 + ty * RPT + q, a.ny - 1);
-        off_coeff[q] = gi * a.si + gj * a.sj;
-    }
-
-    T pm[MAIN], ph = T(0), pc[RPT];
-    auto issue = [&](int64_t k) {
-        const T *in_k = inp + k * a.sk;
-#pragma unroll
-        for (int m = 0; m < MAIN; ++m) pm[m] = in_k[off_main[m]];
-        if (has_halo) ph = in_k[off_halo];
-        const T *cf_k = coeff + k * a.sk;
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) pc[q] = cf_k[off_coeff[q]];
-    };
-    auto commit = [&](int buf) {
-#pragma unroll
-        for (int m = 0; m < MAIN; ++m)
-            if (lds_main[m] >= 0) s_in[buf][lds_main[m]] = C(pm[m]);
-        if (has_halo) s_in[buf][lds_halo] = C(ph);
-    };
-
-    if (k_begin >= k_end) return;
-    issue(k_begin);
-    commit(0);
-    const int col = tx + 2;         // staged column of this thread
-    const int row0 = ty * RPT + 2;  // staged row of its first output
-    for (int64_t k = k_begin; k < k_end; ++k) {
-        const int cur = int(k - k_begin) & 1;
-        C cf[RPT];
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) cf[q] = C(pc[q]);
-        __syncthreads();
-        const bool more = k + 1 < k_end;
-        if (more) issue(k + 1);
-
-        const C *sin = s_in[cur];
-        C ic[RPT + 4], il[RPT + 2], ir[RPT + 2], lap[RPT + 2];
-#pragma unroll
-        for (int q = 0; q < RPT + 4; ++q) ic[q] = sin[(row0 - 2 + q) * LW + col];
-#pragma unroll
-        for (int q = 0; q < RPT + 2; ++q) {
-            il[q] = sin[(row0 - 1 + q) * LW + col - 1];
-            ir[q] = sin[(row0 - 1 + q) * LW + col + 1];
-        }
-#pragma unroll
-        for (int q = 0; q < RPT + 2; ++q)
-            lap[q] = C(4) * ic[q + 1] - (((ir[q] + il[q]) + ic[q + 2]) + ic[q]);
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) s_lap[(ty * RPT + q) * PW + tx + 1] = lap[q + 1];
-        if (tid < 2 * TY) {
-            // Laplacian of the tile's halo columns -1 and TX.
-            const int side = tid / TY, r = tid - side * TY;
-            const int c = side ? TX + 2 : 1;
-            const int rr = r + 2;
-            const C v = C(4) * sin[rr * LW + c] -
-                        (((sin[rr * LW + c + 1] + sin[rr * LW + c - 1]) + sin[(rr + 1) * LW + c]) +
-                         sin[(rr - 1) * LW + c]);
-            s_lap[r * PW + (side ? TX + 1 : 0)] = v;
-        }
-        __syncthreads();
-
-        C res[RPT];
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int r = ty * RPT + q;
-            const C c0 = ic[q + 2];
-            const C lc = lap[q + 1];
-            const C flx = limited(s_lap[r * PW + tx + 2] - lc, ir[q + 1] - c0);
-            const C flx_m = limited(lc - s_lap[r * PW + tx], c0 - il[q + 1]);
-            const C fly = limited(lap[q + 2] - lc, ic[q + 3] - c0);
-            const C fly_m = limited(lc - lap[q], c0 - ic[q + 1]);
-            res[q] = c0 - cf[q] * (((flx - flx_m) + fly) - fly_m);
-        }
-        if (gi_out < a.nx) {
-            T *out_k = out + k * a.sk + gi_out * a.si;
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                const int64_t gj = j0 + ty * RPT + q;
-                if (gj < a.ny) out_k[gj * a.sj] = T(res[q]);
-            }
-        }
-        if (more) commit(cur ^ 1);
-    }
+    const HdiffGeom g{a.nx, a.ny, a.nz, a.si, a.sj, a.sk};
+    Tile::run(inp, coeff, out, g, i0, j0, k_begin, k_end, s_in, s_lap);
 }
 
 template <typename T, typename C>
@@ -210,36 +92,44 @@ __global__ __launch_bounds__(256) void hdiff_otf_kernel(const T *__restrict__ in
 }
 
 template <int TY, int RPT, int KC, typename T, typename C>
-hipError_t launch_lds(const T *inp, const T *coeff, T *out, HdiffArgs a, Context *ctx) {
+hipError_t launch_lds(const T *inp, const T *coeff, T *out, HdiffArgs a, Context *ctx, int flags) {
     a.tiles_i = int32_t((a.nx + 63) / 64);
     a.tiles_j = int32_t((a.ny + TY - 1) / TY);
     const uint64_t blocks =
         uint64_t(a.tiles_i) * uint64_t(a.tiles_j) * uint64_t((a.nz + KC - 1) / KC);
-    hipLaunchKernelGGL((hdiff_lds_kernel<T, C, TY, RPT, KC>), dim3(unsigned(blocks)),
-                       dim3(64, TY / RPT), 0, ctx->stream, inp, coeff, out, a);
+    switch (flags & 3) {
+        case 0: hipLaunchKernelGGL((hdiff_lds_kernel<T, C, TY, RPT, KC, 0>), dim3(unsigned(blocks)),
+                                   dim3(64, TY / RPT), 0, ctx->stream, inp, coeff, out, a); break;
+        case 1: hipLaunchKernelGGL((hdiff_lds_kernel<T, C, TY, RPT, KC, 1>), dim3(unsigned(blocks)),
+                                   dim3(64, TY / RPT), 0, ctx->stream, inp, coeff, out, a); break;
+        case 2: hipLaunchKernelGGL((hdiff_lds_kernel<T, C, TY, RPT, KC, 2>), dim3(unsigned(blocks)),
+                                   dim3(64, TY / RPT), 0, ctx->stream, inp, coeff, out, a); break;
+        default: hipLaunchKernelGGL((hdiff_lds_kernel<T, C, TY, RPT, KC, 3>), dim3(unsigned(blocks)),
+                                    dim3(64, TY / RPT), 0, ctx->stream, inp, coeff, out, a); break;
+    }
     return hipGetLastError();
 }
 
 template <int TY, int RPT, typename T, typename C>
 hipError_t dispatch_kc_ty(int kc, const T *inp, const T *coeff, T *out, HdiffArgs a,
-                          Context *ctx) {
+                          Context *ctx, int flags) {
     switch (kc) {
-        case 1: return launch_lds<TY, RPT, 1, T, C>(inp, coeff, out, a, ctx);
-        case 2: return launch_lds<TY, RPT, 2, T, C>(inp, coeff, out, a, ctx);
-        case 4: return launch_lds<TY, RPT, 4, T, C>(inp, coeff, out, a, ctx);
-        case 8: return launch_lds<TY, RPT, 8, T, C>(inp, coeff, out, a, ctx);
-        case 16: return launch_lds<TY, RPT, 16, T, C>(inp, coeff, out, a, ctx);
+        case 1: return launch_lds<TY, RPT, 1, T, C>(inp, coeff, out, a, ctx, flags);
+        case 2: return launch_lds<TY, RPT, 2, T, C>(inp, coeff, out, a, ctx, flags);
+        case 4: return launch_lds<TY, RPT, 4, T, C>(inp, coeff, out, a, ctx, flags);
+        case 8: return launch_lds<TY, RPT, 8, T, C>(inp, coeff, out, a, ctx, flags);
+        case 16: return launch_lds<TY, RPT, 16, T, C>(inp, coeff, out, a, ctx, flags);
         default: return hipErrorInvalidValue;
     }
 }
 
 template <typename T, typename C>
 hipError_t dispatch_kc(int ty, int kc, const T *inp, const T *coeff, T *out, HdiffArgs a,
-                       Context *ctx) {
+                       Context *ctx, int flags) {
     switch (ty) {
-        case 8: return dispatch_kc_ty<8, 2, T, C>(kc, inp, coeff, out, a, ctx);
-        case 16: return dispatch_kc_ty<16, 4, T, C>(kc, inp, coeff, out, a, ctx);
-        case 32: return dispatch_kc_ty<32, 8, T, C>(kc, inp, coeff, out, a, ctx);
+        case 8: return dispatch_kc_ty<8, 2, T, C>(kc, inp, coeff, out, a, ctx, flags);
+        case 16: return dispatch_kc_ty<16, 4, T, C>(kc, inp, coeff, out, a, ctx, flags);
+        case 32: return dispatch_kc_ty<32, 8, T, C>(kc, inp, coeff, out, a, ctx, flags);
         default: return hipErrorInvalidValue;
     }
 }
@@ -262,7 +152,11 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
     HdiffArgs a{g->nx, g->ny, g->nz, g->si, g->sj, g->sk, 0, 0, ctx->xcds, 0};
     const int variant = l ? l->variant : 0;
     const int ty = (l && l->by) ? l->by : 16;
-    const int kc = (l && l->bz) ? l->bz : 8;
+    // defaults measured on MI355X (512 x 512 x 160): 64 x 16 tiles, 2 planes
+    // per workgroup in FP64 arithmetic, 8 with FP32 arithmetic
+    const int kc = (l && l->bz) ? l->bz : (sizeof(C) == 8 && sizeof(T) == 8 ? 2 : 8);
+    // launch->unroll: bit 0 non-temporal out stores, bit 1 non-temporal coeff loads
+    const int flags = l ? (l->unroll & 3) : 0;
     SB_REQUIRE(variant == 0 || variant == 1 || variant == 3, "hdiff variant %d not available",
                variant);
     if (variant == 3) {
@@ -280,7 +174,7 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
                             uint64_t((g->nz + kc - 1) / kc);
     SB_REQUIRE(blocks < (uint64_t(1) << 31), "domain too large for one launch");
     return timed_launch(l, seconds, [&]() -> hipError_t {
-        return dispatch_kc<T, C>(ty, kc, inp, coeff, out, a, ctx);
+        return dispatch_kc<T, C>(ty, kc, inp, coeff, out, a, ctx, flags);
     });
 }
 

@@ -1,0 +1,179 @@
+// Horizontal diffusion tile worker shared by the plain kernel (hdiff.hip)
+// and the batched persistent kernel (batch.hip).  See hdiff.hip for the
+// arithmetic and the tile schedule.
+#pragma once
+
+#include "sb_internal.h"
+
+namespace sb {
+namespace {
+
+struct HdiffGeom {
+    int64_t nx, ny, nz, si, sj, sk;
+};
+
+template <typename C>
+__device__ __forceinline__ C limited(C flux, C gradient) {
+    return flux * gradient > C(0) ? C(0) : flux;
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_nt(const T *p) {
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ void st_nt(T *p, T v) {
+    if constexpr (NT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+// Tile worker: the calling workgroup (64 x TY/RPT threads) computes the
+// 64 x TY output tile at (i0, j0) for planes [k_begin, k_end).  Staging:
+// each of the NW waves loads rows w, w+NW, ... of the tile's 64 main
+// columns (one coalesced 64-element row segment per load), and threads
+// 0..4*LH-1 load the four halo columns; out-of-range indices are clamped
+// into the 2-point halo, so every load is in bounds and the clamped
+// duplicates only feed outputs outside the domain, which are not stored.
+// The loads for plane k+1 are issued into registers before plane k is
+// computed and written to the other half of the double-buffered LDS tile
+// afterwards.  FLAGS bit 0: non-temporal out stores; bit 1: non-temporal
+// coeff loads (neither is re-read; inp is, by the neighbouring halos).
+template <typename T, typename C, int TY, int RPT, int FLAGS>
+struct HdiffTile {
+    static constexpr int TX = 64;
+    static constexpr int NW = TY / RPT;   // waves per workgroup
+    static constexpr int LW = TX + 4;     // staged columns: tile i = -2 .. TX+1
+    static constexpr int LH = TY + 4;     // staged rows:    tile j = -2 .. TY+1
+    static constexpr int PW = TX + 2;     // published Laplacian columns: -1 .. TX
+    static constexpr int MAIN = (LH + NW - 1) / NW;
+    static constexpr int IN_ELEMS = 2 * LH * LW;  // double-buffered input tile
+    static constexpr int LAP_ELEMS = TY * PW;
+
+    __device__ static void run(const T *__restrict__ inp, const T *__restrict__ coeff,
+                               T *__restrict__ out, const HdiffGeom &a, int64_t i0, int64_t j0,
+                               int64_t k_begin, int64_t k_end, C *s_in_base, C *s_lap) {
+        C(*s_in)[LH * LW] = reinterpret_cast<C(*)[LH * LW]>(s_in_base);
+        const int tx = threadIdx.x, ty = threadIdx.y;
+        const int tid = ty * TX + tx;
+        // ---- per-thread staging offsets (plane-independent) -----------------
+        int64_t off_main[MAIN];
+        int lds_main[MAIN];
+    #pragma unroll
+        for (int m = 0; m < MAIN; ++m) {
+            const int r = ty + NW * m;
+            const int rr = r < LH ? r : LH - 1;
+            const int64_t gi = min(i0 + tx, a.nx + 1);
+            const int64_t gj = min(j0 + rr - 2, a.ny + 1);
+            off_main[m] = gi * a.si + gj * a.sj;
+            lds_main[m] = r < LH ? rr * LW + tx + 2 : -1;
+        }
+        const bool has_halo = tid < 4 * LH;
+        int64_t off_halo = 0;
+        int lds_halo = -1;
+        if (has_halo) {
+            const int r = tid >> 2, q = tid & 3;
+            const int c = q < 2 ? q : TX + q;  // 0, 1, TX+2, TX+3
+            const int64_t gi = min(i0 + c - 2, a.nx + 1);
+            const int64_t gj = min(j0 + r - 2, a.ny + 1);
+            off_halo = gi * a.si + gj * a.sj;
+            lds_halo = r * LW + c;
+        }
+        const int64_t gi_out = i0 + tx;
+        int64_t off_coeff[RPT];
+    #pragma unroll
+        for (int q = 0; q < RPT; ++q) {
+            const int64_t gi = min(gi_out, a.nx - 1);
+            const int64_t gj = min(j0 + ty * RPT + q, a.ny - 1);
+            off_coeff[q] = gi * a.si + gj * a.sj;
+        }
+
+        T pm[MAIN], ph = T(0), pc[RPT];
+        auto issue = [&](int64_t k) {
+            const T *in_k = inp + k * a.sk;
+    #pragma unroll
+            for (int m = 0; m < MAIN; ++m) pm[m] = in_k[off_main[m]];
+            if (has_halo) ph = in_k[off_halo];
+            const T *cf_k = coeff + k * a.sk;
+    #pragma unroll
+            for (int q = 0; q < RPT; ++q) pc[q] = ld_nt<(FLAGS & 2) != 0>(cf_k + off_coeff[q]);
+        };
+        auto commit = [&](int buf) {
+    #pragma unroll
+            for (int m = 0; m < MAIN; ++m)
+                if (lds_main[m] >= 0) s_in[buf][lds_main[m]] = C(pm[m]);
+            if (has_halo) s_in[buf][lds_halo] = C(ph);
+        };
+
+        if (k_begin >= k_end) return;
+        issue(k_begin);
+        commit(0);
+        const int col = tx + 2;         // staged column of this thread
+        const int row0 = ty * RPT + 2;  // staged row of its first output
+        for (int64_t k = k_begin; k < k_end; ++k) {
+            const int cur = int(k - k_begin) & 1;
+            C cf[RPT];
+    #pragma unroll
+            for (int q = 0; q < RPT; ++q) cf[q] = C(pc[q]);
+            __syncthreads();
+            const bool more = k + 1 < k_end;
+            if (more) issue(k + 1);
+
+            const C *sin = s_in[cur];
+            C ic[RPT + 4], il[RPT + 2], ir[RPT + 2], lap[RPT + 2];
+    #pragma unroll
+            for (int q = 0; q < RPT + 4; ++q) ic[q] = sin[(row0 - 2 + q) * LW + col];
+    #pragma unroll
+            for (int q = 0; q < RPT + 2; ++q) {
+                il[q] = sin[(row0 - 1 + q) * LW + col - 1];
+                ir[q] = sin[(row0 - 1 + q) * LW + col + 1];
+            }
+    #pragma unroll
+            for (int q = 0; q < RPT + 2; ++q)
+                lap[q] = C(4) * ic[q + 1] - (((ir[q] + il[q]) + ic[q + 2]) + ic[q]);
+    #pragma unroll
+            for (int q = 0; q < RPT; ++q) s_lap[(ty * RPT + q) * PW + tx + 1] = lap[q + 1];
+            if (tid < 2 * TY) {
+                // Laplacian of the tile's halo columns -1 and TX.
+                const int side = tid / TY, r = tid - side * TY;
+                const int c = side ? TX + 2 : 1;
+                const int rr = r + 2;
+                const C v = C(4) * sin[rr * LW + c] -
+                            (((sin[rr * LW + c + 1] + sin[rr * LW + c - 1]) + sin[(rr + 1) * LW + c]) +
+                             sin[(rr - 1) * LW + c]);
+                s_lap[r * PW + (side ? TX + 1 : 0)] = v;
+            }
+            __syncthreads();
+
+            C res[RPT];
+    #pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int r = ty * RPT + q;
+                const C c0 = ic[q + 2];
+                const C lc = lap[q + 1];
+                const C flx = limited(s_lap[r * PW + tx + 2] - lc, ir[q + 1] - c0);
+                const C flx_m = limited(lc - s_lap[r * PW + tx], c0 - il[q + 1]);
+                const C fly = limited(lap[q + 2] - lc, ic[q + 3] - c0);
+                const C fly_m = limited(lc - lap[q], c0 - ic[q + 1]);
+                res[q] = c0 - cf[q] * (((flx - flx_m) + fly) - fly_m);
+            }
+            if (gi_out < a.nx) {
+                T *out_k = out + k * a.sk + gi_out * a.si;
+    #pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int64_t gj = j0 + ty * RPT + q;
+                    if (gj < a.ny) st_nt<(FLAGS & 1) != 0>(out_k + gj * a.sj, T(res[q]));
+                }
+            }
+            if (more) commit(cur ^ 1);
+        }
+    }
+};
+
+}  // namespace
+}  // namespace sb

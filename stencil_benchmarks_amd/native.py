@@ -92,6 +92,10 @@ SIGNATURES = {
     "sb_stream_fill_random": [_I, _P, _SZ, ctypes.c_uint64],
     "sb_stream_kernel": [_I, _I, _P, _P, _P, _SZ, _D, _LP, _DP],
     "sb_stream_check": [_I, _P, _SZ, _D, _DP],
+    "sb_batch_step": [_I, _P, _I, _LP, _LP, _DP],
+    "sb_batch_plan_create": [_I, _P, _I, _I, _LP, _LP, ctypes.POINTER(_P)],
+    "sb_batch_plan_run": [_P, _I, _DP],
+    "sb_batch_plan_destroy": [_P],
 }
 
 _lock = threading.Lock()
