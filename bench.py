@@ -130,7 +130,7 @@ def bench_stream(steps):
     words = {"copy": 2, "scale": 2, "add": 3, "triad": 3}
     for variant in (0, 3):
         for op, code in native.STREAM_OPS.items():
-            opts = native.launch_options(variant=variant, block=(256, 1, 1), unroll=4)
+            opts = native.launch_options(variant=variant, block=(256, 1, 1), unroll=1)
             best = None
             for _ in range(max(3, steps)):
                 seconds = ctypes.c_double(0.0)
@@ -183,6 +183,54 @@ def bench_hdiff_f32(steps):
     return hdiff_bytes(domain, 4) / times.mean() / 1e9, times.mean()
 
 
+def bench_batched(compute_dtype, storage_dtype, repeats=2):
+    """96 uneven subdomains x 10 steps of hdiff then vadv, one hipGraph."""
+    from stencil_benchmarks_amd import batch
+
+    specs = batch.random_specs(96, 32, 512, 40, 160, seed=0)
+    subs = [batch.Subdomain(s.domain, s.halo, storage_dtype, seed=s.seed) for s in specs]
+    plan = batch.Plan(subs, steps=10, compute=compute_dtype)
+    plan.run(use_graph=True)  # capture + warm-up
+    native.call("sb_flush_caches")
+    times = [plan.run(use_graph=True) for _ in range(repeats)]
+    seconds = min(times)
+    nbytes = 10 * sum(s.compulsory_bytes_per_step() for s in subs)
+    points = 10 * sum(s.points for s in subs)
+    plan.close()
+    for s in subs:
+        s.free()
+    return {"seconds": seconds, "gbs": nbytes / seconds / 1e9,
+            "gridpoint_steps_per_s": points / seconds, "subdomains": len(subs),
+            "steps": 10, "points_per_step": points // 10}
+
+
+def bench_serving(count=160, load=0.8):
+    """Poisson stream of hdiff+vadv requests, float32 storage, FP64 arithmetic."""
+    from stencil_benchmarks_amd import serve
+
+    mix = serve.poisson_requests(rate=0.0, count=count, seed=1)
+    warm = serve.Server(dtype="float32", compute="float64")
+    warm.run(serve.poisson_requests(rate=0.0, count=8, seed=99), realtime=False)
+    closed = serve.Server(dtype="float32", compute="float64").run(mix, realtime=False)
+    capacity = closed.requests / closed.wall_s
+    rate = load * capacity
+    stream = serve.poisson_requests(rate=rate, count=count, seed=1)
+    served = serve.Server(dtype="float32", compute="float64").run(stream, realtime=True)
+    plain_s, plain_bytes, plain_points = serve.run_plain(
+        serve.poisson_requests(rate=0.0, count=count, seed=1), dtype="float64")
+    out = served.summary()
+    out.update({
+        "rate_requests_per_s": rate,
+        "load_fraction_of_capacity": load,
+        "capacity_requests_per_s": capacity,
+        "plain_f64_requests_per_s": count / plain_s,
+        "plain_f64_gbs": plain_bytes / plain_s / 1e9,
+        "speedup_vs_plain_f64": (served.requests / served.wall_s) / (count / plain_s),
+        "closed_loop_speedup_vs_plain_f64": capacity / (count / plain_s),
+    })
+    return out
+
+
 def cpu_baseline(headline, budget_s=12.0):
     """OpenMP port of the reference CPU hdiff on the headline fields."""
     from oracle import oracle
@@ -200,7 +248,7 @@ def cpu_baseline(headline, budget_s=12.0):
     lib.cpu_hdiff_f64(*args, ctypes.c_int(threads))  # warm-up (first touch, thread spin-up)
     times = []
     spent = 0.0
-    while spent < budget_s and len(times) < 20:
+    while spent < budget_s and len(times) < 5000:
         t = lib.cpu_hdiff_f64(*args, ctypes.c_int(threads))
         times.append(t)
         spent += t
@@ -225,6 +273,8 @@ def main():
     parser.add_argument("--steps", type=int, default=20)
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--quick", action="store_true", help="headline only")
+    parser.add_argument("--only", choices=["hdiff", "vadv", "stream", "batched", "serving"],
+                        help="run a single workload (profiling)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = parser.parse_args()
     if args.gpus != 1:
@@ -236,6 +286,9 @@ def main():
     info = native.device_info(0)
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']}")
 
+    if args.only and args.only != "hdiff":
+        run_single(args)
+        return
     log("hdiff 512x512x160 float64 (headline)")
     head = bench_hdiff_headline(args.steps, args.warmup)
     mean_s = float(head["times"].mean())
@@ -264,6 +317,18 @@ def main():
         extras["hdiff_frac_of_triad"] = achieved / triad
         extras["vadv_512x512x160_f64_frac_of_triad"] = extras["vadv_512x512x160_f64_gbs"] / triad
         extras["vadv_256x256x80_f64_frac_of_triad"] = extras["vadv_256x256x80_f64_gbs"] / triad
+        log("batched: 96 subdomains x 10 steps (float64, float32 storage)")
+        b64 = bench_batched(np.float64, np.float64)
+        b32 = bench_batched(np.float64, np.float32)
+        extras["batched_f64"] = b64
+        extras["batched_f32s_f64c"] = b32
+        extras["batched_f32_speedup_vs_f64"] = b64["seconds"] / b32["seconds"]
+        log(f"batched f64 {b64['gbs']:.0f} GB/s, f32 {b32['gbs']:.0f} GB/s")
+        log("serving: Poisson stream at 80% of capacity")
+        extras["serving"] = bench_serving()
+        log(f"serving p50 {extras['serving']['p50_latency_ms']:.2f} ms "
+            f"p99 {extras['serving']['p99_latency_ms']:.2f} ms, "
+            f"speedup vs plain f64 {extras['serving']['speedup_vs_plain_f64']:.2f}")
 
     baseline = None
     if not args.no_cpu:
@@ -305,6 +370,21 @@ def main():
         "extras": extras,
     }
     print(json.dumps(record))
+
+
+def run_single(args):
+    """One workload only, printed as JSON (used under rocprofv3)."""
+    out = {"workload": args.only}
+    if args.only == "vadv":
+        out["vadv_512x512x160_f64_gbs"], out["mean_s"] = bench_vadv(
+            (512, 512, 160), np.float64, native.SB_F64, args.steps)
+    elif args.only == "stream":
+        out["stream_gbs"] = bench_stream(args.steps)
+    elif args.only == "batched":
+        out["batched_f64"] = bench_batched(np.float64, np.float64)
+    elif args.only == "serving":
+        out["serving"] = bench_serving(count=60)
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

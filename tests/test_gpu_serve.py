@@ -1,0 +1,64 @@
+"""Serving loop: results equal alone and in the mix; oracle spot check."""
+
+import numpy as np
+import pytest
+
+from conftest import inner
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _requests():
+    from stencil_benchmarks_amd import serve
+
+    reqs = serve.poisson_requests(rate=0.0, count=7, seed=3, nmin=8, nmax=96,
+                                  nz_choices=(2, 9, 40), smin=1, smax=4)
+    return reqs
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_serve_results_alone_equal_mix(gpu, dtype):
+    from stencil_benchmarks_amd import serve
+
+    mixed = serve.Server(dtype=dtype, keep_results=True)
+    stats = mixed.run(_requests(), realtime=False)
+    assert stats.requests == 7 and stats.max_active > 1
+    for req in _requests():
+        alone = serve.Server(dtype=dtype, keep_results=True)
+        alone.run([req], realtime=False)
+        u_mix, ts_mix = mixed.results[req.rid]
+        u_one, ts_one = alone.results[req.rid]
+        assert np.array_equal(u_mix, u_one) and np.array_equal(ts_mix, ts_one), req
+
+
+def test_serve_matches_oracle(gpu):
+    from stencil_benchmarks_amd import batch, serve
+    from test_gpu_batch import oracle_step
+
+    req = _requests()[0]
+    server = serve.Server(dtype="float64", keep_results=True)
+    server.run([req], realtime=False)
+    # rebuild the request's initial fields on the device and read them back
+    sub = batch.Subdomain(req.domain, (3, 3, 3), "float64", seed=req.seed)
+    fields = {"u": sub.download("u0")}
+    for name in ("coeff", "upos", "utens", "utensstage", "wcon"):
+        fields[name] = sub.download(name)
+    for _ in range(req.steps):
+        oracle_step(fields, req.domain, (3, 3, 3), np.float64)
+    sl = inner(req.domain, (3, 3, 3))
+    u, ts = server.results[req.rid]
+    assert np.array_equal(u, fields["u"][sl]) and np.array_equal(ts, fields["utensstage"][sl])
+
+
+@pytest.mark.gpu
+def test_poisson_stream_shape():
+    from stencil_benchmarks_amd import serve
+
+    reqs = serve.poisson_requests(rate=100.0, count=2000, seed=0)
+    sizes = np.array([r.domain[0] for r in reqs])
+    assert sizes.min() >= 16 and sizes.max() <= 1024
+    assert {r.domain[2] for r in reqs} == {40, 80, 137}
+    gaps = np.diff([0.0] + [r.arrival for r in reqs])
+    assert abs(gaps.mean() - 0.01) < 0.002
+    assert 1 <= min(r.steps for r in reqs) and max(r.steps for r in reqs) <= 20
