@@ -14,10 +14,15 @@
 // cost, largest first, so the tail of the queue holds the small items.
 //   hdiff items: (subdomain, 64 x TY tile, KC-plane chunk), the tile worker
 //     of the plain kernel (hdiff_tile.h);
-//   vadv items:  a run of up to CHUNK consecutive 64-column groups of one
-//     subdomain, streamed through the fused forward/backward pipeline of
-//     the plain kernel (vadv_column.h) with an LDS buffer sized for the
-//     tallest column in the batch.
+//   vadv items:  (subdomain, residue r): the 64-column groups r, r + m,
+//     r + 2m, ... of one subdomain, where m (the subdomain's item count) is
+//     at most the number of resident workgroups, streamed through the fused
+//     forward/backward pipeline of the plain kernel (vadv_column.h) with an
+//     LDS buffer sized for the tallest column in the batch.  Workgroups that
+//     take consecutive items of one subdomain walk adjacent groups level by
+//     level, so the waves of the chip touch the same few k planes (and
+//     pages) at a time, as in the plain kernel; runs of consecutive groups
+//     per workgroup scatter the waves over many planes and ran 3x slower.
 // The counters are cleared by a small kernel at the start of every
 // sequence (no memset nodes, so the sequence can be captured in a
 // hipGraph and replayed).  Every workgroup exits once the counter passes
@@ -33,7 +38,7 @@
 namespace sb {
 namespace {
 
-constexpr int CHUNK = 32;  // column groups per vadv work item
+constexpr int kMinGroupsPerItem = 8;  // amortises the pipeline's fill and drain
 
 // Device descriptor of one subdomain for one step parity.
 struct SubDev {
@@ -46,8 +51,19 @@ struct SubDev {
     int64_t nx, ny, nz, si, sj, sk;
     int32_t tiles_i, tiles_j;
     uint32_t span_bytes;
-    int32_t pad;
+    int32_t vstride;  // vadv items of this subdomain = group stride
 };
+
+// A wave-uniform copy of lane 0's value.  Work-queue tickets reach the other
+// lanes through this (not a shuffle or an LDS read), so the compiler knows
+// the subdomain index and its descriptor are uniform: descriptor fields stay
+// in SGPRs and buffer resources need no per-lane waterfall loop around every
+// load and store.
+__device__ __forceinline__ int64_t uniform64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(v));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(v >> 32));
+    return int64_t((uint64_t(hi) << 32) | lo);
+}
 
 __device__ __forceinline__ int find_sub(const int64_t *prefix, int count, int64_t item) {
     // largest s with prefix[s] <= item (prefix[0] = 0, prefix[count] = total)
@@ -80,7 +96,7 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
     for (;;) {
         if (tid == 0) s_item = (long long)atomicAdd(counter, 1ull);
         __syncthreads();
-        const int64_t item = s_item;
+        const int64_t item = uniform64(uint64_t(s_item));
         __syncthreads();
         if (item >= total) break;
         const int s = find_sub(prefix, count, item);
@@ -94,8 +110,11 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
         const int64_t k_begin = kc * KC;
         const int64_t k_end = min(k_begin + int64_t(KC), d.nz);
         const HdiffGeom g{d.nx, d.ny, d.nz, d.si, d.sj, d.sk};
-        Tile::run(static_cast<const T *>(d.inp), static_cast<const T *>(d.coeff),
-                  static_cast<T *>(d.out), g, i0, j0, k_begin, k_end, s_in, s_lap);
+        // descriptor pointers have no known address space: pass them as global
+        // pointers so the tile's loads are global (flat loads would also count
+        // against the LDS wait counter)
+        Tile::run(as_global(static_cast<const T *>(d.inp)), as_global(static_cast<const T *>(d.coeff)),
+                  as_global(static_cast<T *>(d.out)), g, i0, j0, k_begin, k_end, s_in, s_lap);
     }
 }
 
@@ -110,13 +129,14 @@ void vadv_batch_kernel(const SubDev *__restrict__ subs, const int64_t *__restric
     for (;;) {
         unsigned long long ticket = 0;
         if (threadIdx.x == 0) ticket = atomicAdd(counter, 1ull);
-        const int64_t item = int64_t(__shfl(ticket, 0));
+        const int64_t item = uniform64(ticket);
         if (item >= total) break;
         const int s = find_sub(prefix, count, item);
         const SubDev d = subs[s];
         const int64_t groups = int64_t(d.tiles_i) * d.ny;
-        const int64_t g0 = (item - prefix[s]) * CHUNK;
-        const int64_t n = min(int64_t(CHUNK), groups - g0);
+        const int64_t m = d.vstride;
+        const int64_t g0 = item - prefix[s];
+        const int64_t n = (groups - g0 + m - 1) / m;  // groups g0, g0 + m, ...
         const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, d.tiles_i, 1, groups,
                          d.span_bytes, 0};
         const Rsrc R{make_rsrc(d.out, d.span_bytes), make_rsrc(d.pos, d.span_bytes),
@@ -127,10 +147,10 @@ void vadv_batch_kernel(const SubDev *__restrict__ subs, const int64_t *__restric
         T last_pos = T(0);
         pipe_iteration<T, C, PF, AUX, true, false>(R, wshift, a, g0, 0, false, lds, x, last_pos);
         for (int64_t q = 1; q < n; ++q)
-            pipe_iteration<T, C, PF, AUX, true, true>(R, wshift, a, g0 + q, g0 + q - 1,
+            pipe_iteration<T, C, PF, AUX, true, true>(R, wshift, a, g0 + q * m, g0 + (q - 1) * m,
                                                       (q & 1) != 0, lds, x, last_pos);
-        pipe_iteration<T, C, 32, AUX, false, true>(R, wshift, a, 0, g0 + n - 1, (n & 1) != 0, lds,
-                                                   x, last_pos);
+        pipe_iteration<T, C, 32, AUX, false, true>(R, wshift, a, 0, g0 + (n - 1) * m, (n & 1) != 0,
+                                                   lds, x, last_pos);
     }
 }
 
@@ -156,8 +176,10 @@ struct Plan {
 };
 
 int hdiff_ty(const sb_launch *l) { return (l && l->by) ? l->by : 16; }
-int hdiff_kc(const sb_launch *l, int prec) {
-    return (l && l->bz) ? l->bz : (prec == SB_F64 ? 2 : 8);
+int hdiff_kc(const sb_launch *l, int) {
+    // 8 planes per item measured best for every precision on the 96-subdomain
+    // batch (32..512 squared, nz 40..160)
+    return (l && l->bz) ? l->bz : 8;
 }
 int hdiff_flags(const sb_launch *l) { return l && l->unroll ? (l->unroll & 3) : 2; }
 int vadv_pf(const sb_launch *l) { return (l && l->unroll) ? l->unroll : 8; }
@@ -185,8 +207,25 @@ int validate(int prec, const sb_subdomain *subs, int count) {
     return 0;
 }
 
+template <typename C>
+size_t vadv_lds_bytes(int max_nz) {
+    return size_t(max_nz - 1) * 64 * sizeof(CD<C>);
+}
+
+size_t plan_vadv_lds(int prec, int max_nz) {
+    return prec == SB_F32 ? vadv_lds_bytes<float>(max_nz) : vadv_lds_bytes<double>(max_nz);
+}
+
+// Resident vadv workgroups (one 64-lane wave each) for the tallest column.
+unsigned vadv_workers(const Plan &p, const Context *ctx) {
+    const size_t lds = plan_vadv_lds(p.prec, p.max_nz);
+    const int fit = std::max(1, int(ctx->lds_per_block / std::max<size_t>(lds, 1)));
+    return unsigned(fit * ctx->compute_units);
+}
+
 // Build descriptors and prefix sums (largest subdomains first).
-void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch *hl, Plan &p) {
+void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch *hl, Plan &p,
+                  const Context *ctx) {
     const size_t esize = prec == SB_F64 ? 8 : 4;
     const int ty = hdiff_ty(hl), kc = hdiff_kc(hl, prec);
     std::vector<int> order(count);
@@ -197,6 +236,8 @@ void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost(a) > cost(b); });
     p.max_nz = 2;
+    for (int i = 0; i < count; ++i) p.max_nz = std::max<int>(p.max_nz, int(subs[i].geom.nz));
+    const int64_t workers = vadv_workers(p, ctx);
     for (int flip = 0; flip < 2; ++flip) p.host_subs[flip].resize(count);
     p.host_hprefix.assign(count + 1, 0);
     p.host_vprefix.assign(count + 1, 0);
@@ -205,6 +246,9 @@ void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch
         const sb_geom &g = s.geom;
         const int64_t last = (g.nx - 1 + g.hi) * g.si + (g.ny - 1 + g.hj) * g.sj +
                              (g.nz - 1 + g.hk) * g.sk;
+        const int64_t tiles_i = (g.nx + 63) / 64;
+        const int64_t groups = tiles_i * g.ny;
+        const int64_t vitems = std::max<int64_t>(1, std::min(workers, groups / kMinGroupsPerItem));
         for (int flip = 0; flip < 2; ++flip) {
             const int cur = (s.parity + flip) & 1;
             SubDev d{};
@@ -221,23 +265,16 @@ void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch
             d.si = g.si;
             d.sj = g.sj;
             d.sk = g.sk;
-            d.tiles_i = int32_t((g.nx + 63) / 64);
+            d.tiles_i = int32_t(tiles_i);
             d.tiles_j = int32_t((g.ny + ty - 1) / ty);
             d.span_bytes = uint32_t(uint64_t(last + 1) * esize);
+            d.vstride = int32_t(vitems);
             p.host_subs[flip][r] = d;
         }
-        const int64_t tiles_i = (g.nx + 63) / 64;
         const int64_t hitems = tiles_i * ((g.ny + ty - 1) / ty) * ((g.nz + kc - 1) / kc);
-        const int64_t vitems = (tiles_i * g.ny + CHUNK - 1) / CHUNK;
         p.host_hprefix[r + 1] = p.host_hprefix[r] + hitems;
         p.host_vprefix[r + 1] = p.host_vprefix[r] + vitems;
-        p.max_nz = std::max<int>(p.max_nz, int(g.nz));
     }
-}
-
-template <typename T, typename C>
-size_t vadv_lds_bytes(int max_nz) {
-    return size_t(max_nz - 1) * 64 * sizeof(CD<C>);
 }
 
 // Enqueue one step (both kernels) on ctx->stream.
@@ -267,9 +304,8 @@ hipError_t enqueue_step(const Plan &p, int flip, unsigned long long *counters, C
 #undef SB_HB
     if (return_h != hipSuccess) return return_h;
     // vadv
-    const size_t lds = vadv_lds_bytes<T, C>(p.max_nz);
-    const int fit = std::max(1, int(ctx->lds_per_block / lds));
-    const unsigned vgrid = unsigned(fit * ctx->compute_units);
+    const size_t lds = vadv_lds_bytes<C>(p.max_nz);
+    const unsigned vgrid = vadv_workers(p, ctx);
     auto launch_v = [&](auto kernel) -> hipError_t {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
@@ -329,8 +365,7 @@ int upload_tables(Plan &p, Context *ctx) {
 }
 
 int check_lds(const Plan &p, Context *ctx) {
-    const size_t lds = p.prec == SB_F32 ? vadv_lds_bytes<float, float>(p.max_nz)
-                                        : vadv_lds_bytes<double, double>(p.max_nz);
+    const size_t lds = plan_vadv_lds(p.prec, p.max_nz);
     SB_REQUIRE(lds <= ctx->lds_per_block,
                "tallest column nz = %d needs %zu bytes of LDS, device allows %zu", p.max_nz, lds,
                ctx->lds_per_block);
@@ -398,7 +433,7 @@ int sb_batch_plan_create(int prec, const sb_subdomain *subs, int count, int step
     p->steps = steps;
     if (hdiff_launch) p->hl = *hdiff_launch;
     if (vadv_launch) p->vl = *vadv_launch;
-    build_tables(prec, subs, count, hdiff_launch, *p);
+    build_tables(prec, subs, count, hdiff_launch, *p, ctx);
     if (check_lds(*p, ctx)) return 2;
     if (upload_tables(*p, ctx)) {
         destroy_plan(p.release());
@@ -434,7 +469,7 @@ int sb_batch_step(int prec, const sb_subdomain *subs, int count, const sb_launch
     p->steps = 1;
     p->hl = hdiff_launch ? *hdiff_launch : sb_launch{};
     p->vl = vadv_launch ? *vadv_launch : sb_launch{};
-    build_tables(prec, subs, count, hdiff_launch, *p);
+    build_tables(prec, subs, count, hdiff_launch, *p, ctx);
     if (check_lds(*p, ctx)) return 2;
     if (upload_tables(*p, ctx)) return 1;
     return run_plan(*p, 0, seconds);

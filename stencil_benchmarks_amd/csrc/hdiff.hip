@@ -64,7 +64,8 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_lds_kernel(
     const int64_t k_begin = int64_t(kc) * KC;
     const int64_t k_end = min(k_begin + int64_t(KC), a.nz);
     const HdiffGeom g{a.nx, a.ny, a.nz, a.si, a.sj, a.sk};
-    Tile::run(inp, coeff, out, g, i0, j0, k_begin, k_end, s_in, s_lap);
+    Tile::run(as_global(inp), as_global(coeff), as_global(out), g, i0, j0, k_begin, k_end, s_in,
+              s_lap);
 }
 
 template <typename T, typename C>

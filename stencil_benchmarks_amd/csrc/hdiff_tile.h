@@ -8,6 +8,17 @@
 namespace sb {
 namespace {
 
+// Pointer into device global memory.  The tile worker takes its fields as
+// such, so its loads and stores compile to global (not flat) instructions
+// whatever the caller's pointers were read from.
+template <typename T>
+using global_ptr = __attribute__((address_space(1))) T *;
+
+template <typename T>
+__device__ __forceinline__ global_ptr<T> as_global(T *p) {
+    return (global_ptr<T>)p;
+}
+
 struct HdiffGeom {
     int64_t nx, ny, nz, si, sj, sk;
 };
@@ -18,7 +29,7 @@ __device__ __forceinline__ C limited(C flux, C gradient) {
 }
 
 template <bool NT, typename T>
-__device__ __forceinline__ T ld_nt(const T *p) {
+__device__ __forceinline__ T ld_nt(global_ptr<const T> p) {
     if constexpr (NT)
         return __builtin_nontemporal_load(p);
     else
@@ -26,7 +37,7 @@ __device__ __forceinline__ T ld_nt(const T *p) {
 }
 
 template <bool NT, typename T>
-__device__ __forceinline__ void st_nt(T *p, T v) {
+__device__ __forceinline__ void st_nt(global_ptr<T> p, T v) {
     if constexpr (NT)
         __builtin_nontemporal_store(v, p);
     else
@@ -55,8 +66,9 @@ struct HdiffTile {
     static constexpr int IN_ELEMS = 2 * LH * LW;  // double-buffered input tile
     static constexpr int LAP_ELEMS = TY * PW;
 
-    __device__ static void run(const T *__restrict__ inp, const T *__restrict__ coeff,
-                               T *__restrict__ out, const HdiffGeom &a, int64_t i0, int64_t j0,
+    __device__ static void run(global_ptr<const T> __restrict__ inp,
+                               global_ptr<const T> __restrict__ coeff, global_ptr<T> __restrict__ out,
+                               const HdiffGeom &a, int64_t i0, int64_t j0,
                                int64_t k_begin, int64_t k_end, C *s_in_base, C *s_lap) {
         C(*s_in)[LH * LW] = reinterpret_cast<C(*)[LH * LW]>(s_in_base);
         const int tx = threadIdx.x, ty = threadIdx.y;
@@ -95,11 +107,11 @@ struct HdiffTile {
 
         T pm[MAIN], ph = T(0), pc[RPT];
         auto issue = [&](int64_t k) {
-            const T *in_k = inp + k * a.sk;
+            global_ptr<const T> in_k = inp + k * a.sk;
     #pragma unroll
             for (int m = 0; m < MAIN; ++m) pm[m] = in_k[off_main[m]];
             if (has_halo) ph = in_k[off_halo];
-            const T *cf_k = coeff + k * a.sk;
+            global_ptr<const T> cf_k = coeff + k * a.sk;
     #pragma unroll
             for (int q = 0; q < RPT; ++q) pc[q] = ld_nt<(FLAGS & 2) != 0>(cf_k + off_coeff[q]);
         };
@@ -163,7 +175,7 @@ struct HdiffTile {
                 res[q] = c0 - cf[q] * (((flx - flx_m) + fly) - fly_m);
             }
             if (gi_out < a.nx) {
-                T *out_k = out + k * a.sk + gi_out * a.si;
+                global_ptr<T> out_k = out + k * a.sk + gi_out * a.si;
     #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
                     const int64_t gj = j0 + ty * RPT + q;
