@@ -16,9 +16,12 @@
 //     of the plain kernel (hdiff_tile.h);
 //   vadv items:  (subdomain, residue r): the 64-column groups r, r + m,
 //     r + 2m, ... of one subdomain, where m (the subdomain's item count) is
-//     at most the number of resident workgroups, streamed through the fused
-//     forward/backward pipeline of the plain kernel (vadv_column.h) with an
-//     LDS buffer sized for the tallest column in the batch.  Workgroups that
+//     at most the number of resident workgroups, solved by the
+//     warp-specialised column solver of the plain kernel (vadv_ws.h; the
+//     single-wave pipeline of vadv_column.h with variant 3) with an LDS
+//     buffer sized for the tallest column in the batch.  The warp-specialised
+//     kernel takes items round-robin (item i on workgroup i mod grid): its
+//     LDS is full at nz = 160, which leaves no room to broadcast a ticket.  Workgroups that
 //     take consecutive items of one subdomain walk adjacent groups level by
 //     level, so the waves of the chip touch the same few k planes (and
 //     pages) at a time, as in the plain kernel; runs of consecutive groups
@@ -34,6 +37,7 @@
 
 #include "hdiff_tile.h"
 #include "vadv_column.h"
+#include "vadv_ws.h"
 
 namespace sb {
 namespace {
@@ -154,6 +158,28 @@ void vadv_batch_kernel(const SubDev *__restrict__ subs, const int64_t *__restric
     }
 }
 
+template <typename T, typename C, bool FAST>
+__global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_batch_kernel(
+    const SubDev *__restrict__ subs, const int64_t *__restrict__ prefix, int count) {
+    extern __shared__ unsigned char lds_raw[];
+    C *lane0 = reinterpret_cast<C *>(lds_raw) + (threadIdx.x & 63);
+    const int64_t total = prefix[count];
+    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
+        const int s = find_sub(prefix, count, item);
+        const SubDev d = subs[s];
+        const int64_t groups = int64_t(d.tiles_i) * d.ny;
+        const int64_t m = d.vstride;
+        const int64_t r = item - prefix[s];
+        const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, d.tiles_i, 1, groups, d.span_bytes, 0};
+        const Rsrc R{make_rsrc(d.out, d.span_bytes), make_rsrc(d.pos, d.span_bytes),
+                     make_rsrc(d.tens, d.span_bytes), make_rsrc(d.ts, d.span_bytes),
+                     make_rsrc(d.wcon, d.span_bytes)};
+        // u component: wcon neighbour at i+1
+        WsSolver<T, C, 0, FAST>::run(R, d.si, a, WsSeq{r, m, (groups - r + m - 1) / m}, lane0,
+                                     lane0 + ws_rows(d.nz) * 64);
+    }
+}
+
 // ---- host side ---------------------------------------------------------------
 
 struct Plan {
@@ -182,8 +208,14 @@ int hdiff_kc(const sb_launch *l, int) {
     return (l && l->bz) ? l->bz : 8;
 }
 int hdiff_flags(const sb_launch *l) { return l && l->unroll ? (l->unroll & 3) : 2; }
+// vadv launch options: variant 4 (default) warp-specialised solver, 3 the
+// single-wave pipeline (unroll = prefetch depth 4 or 8, bz > 0 non-temporal
+// streamed loads); for variant 4, bz bit 2 selects the fast reciprocal
+// (float32 storage with FP64 arithmetic only)
+int vadv_variant(const sb_launch *l) { return (l && l->variant) ? l->variant : 4; }
 int vadv_pf(const sb_launch *l) { return (l && l->unroll) ? l->unroll : 8; }
 int vadv_nt(const sb_launch *l) { return (l && l->bz) ? (l->bz > 0 ? 1 : 0) : 0; }
+bool vadv_fast(const sb_launch *l) { return l && l->bz > 0 && (l->bz & 4); }
 
 int validate(int prec, const sb_subdomain *subs, int count) {
     SB_REQUIRE(prec == SB_F64 || prec == SB_F32 || prec == SB_F32_F64, "invalid precision %d", prec);
@@ -200,7 +232,7 @@ int validate(int prec, const sb_subdomain *subs, int count) {
         SB_REQUIRE(g.si > 0 && g.sj > 0 && g.sk > 0, "subdomain %d: strides must be positive", i);
         const int64_t last = (g.nx - 1 + g.hi) * g.si + (g.ny - 1 + g.hj) * g.sj +
                              (g.nz - 1 + g.hk) * g.sk;
-        SB_REQUIRE(uint64_t(last + 1) * esize < (uint64_t(1) << 32),
+        SB_REQUIRE(uint64_t(last + 1 + g.sk) * esize < kWsDropOffset,
                    "subdomain %d: fields must be below 4 GiB", i);
         SB_REQUIRE(s.parity == 0 || s.parity == 1, "subdomain %d: parity must be 0 or 1", i);
     }
@@ -212,14 +244,19 @@ size_t vadv_lds_bytes(int max_nz) {
     return size_t(max_nz - 1) * 64 * sizeof(CD<C>);
 }
 
-size_t plan_vadv_lds(int prec, int max_nz) {
+size_t plan_vadv_lds(int prec, int max_nz, int variant) {
+    if (variant == 4)
+        return prec == SB_F32 ? ws_lds_bytes<float>(max_nz) : ws_lds_bytes<double>(max_nz);
     return prec == SB_F32 ? vadv_lds_bytes<float>(max_nz) : vadv_lds_bytes<double>(max_nz);
 }
 
-// Resident vadv workgroups (one 64-lane wave each) for the tallest column.
+// vadv workgroups of one launch for the tallest column: as many as LDS lets
+// a CU hold (the warp-specialised kernel: at most 2, its VGPR budget)
 unsigned vadv_workers(const Plan &p, const Context *ctx) {
-    const size_t lds = plan_vadv_lds(p.prec, p.max_nz);
-    const int fit = std::max(1, int(ctx->lds_per_block / std::max<size_t>(lds, 1)));
+    const int variant = vadv_variant(&p.vl);
+    const size_t lds = plan_vadv_lds(p.prec, p.max_nz, variant);
+    int fit = std::max(1, int(ctx->lds_per_block / std::max<size_t>(lds, 1)));
+    if (variant == 4) fit = std::min(fit, 2);
     return unsigned(fit * ctx->compute_units);
 }
 
@@ -304,8 +341,21 @@ hipError_t enqueue_step(const Plan &p, int flip, unsigned long long *counters, C
 #undef SB_HB
     if (return_h != hipSuccess) return return_h;
     // vadv
-    const size_t lds = vadv_lds_bytes<C>(p.max_nz);
     const unsigned vgrid = vadv_workers(p, ctx);
+    if (vadv_variant(&p.vl) == 4) {
+        const size_t wlds = ws_lds_bytes<C>(p.max_nz);
+        auto launch_w = [&](auto kernel) -> hipError_t {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, int(wlds));
+            hipLaunchKernelGGL(kernel, dim3(vgrid), dim3(64 * kWsWaves), wlds, ctx->stream, subs,
+                               p.vprefix, p.count);
+            return hipGetLastError();
+        };
+        if constexpr (sizeof(T) == 4 && sizeof(C) == 8)
+            if (vadv_fast(&p.vl)) return launch_w(vadv_ws_batch_kernel<T, C, true>);
+        return launch_w(vadv_ws_batch_kernel<T, C, false>);
+    }
+    const size_t lds = vadv_lds_bytes<C>(p.max_nz);
     auto launch_v = [&](auto kernel) -> hipError_t {
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
@@ -365,7 +415,11 @@ int upload_tables(Plan &p, Context *ctx) {
 }
 
 int check_lds(const Plan &p, Context *ctx) {
-    const size_t lds = plan_vadv_lds(p.prec, p.max_nz);
+    const int variant = vadv_variant(&p.vl);
+    SB_REQUIRE(variant == 3 || variant == 4, "batched vadv variant %d not available", variant);
+    SB_REQUIRE(!vadv_fast(&p.vl) || (variant == 4 && p.prec == SB_F32_F64),
+               "the fast reciprocal needs variant 4 and float32 storage with FP64 arithmetic");
+    const size_t lds = plan_vadv_lds(p.prec, p.max_nz, variant);
     SB_REQUIRE(lds <= ctx->lds_per_block,
                "tallest column nz = %d needs %zu bytes of LDS, device allows %zu", p.max_nz, lds,
                ctx->lds_per_block);

@@ -36,6 +36,7 @@
 // blocks the compute wave), and after the forward sweep it does the same
 // for the pos lines of the backward substitution.
 #include "vadv_column.h"
+#include "vadv_ws.h"
 
 namespace sb {
 namespace {
@@ -332,6 +333,28 @@ void vadv_pipe_kernel(Components comps, const T *__restrict__ wcon, PipeArgs a) 
     }
 }
 
+// Variant 4: warp-specialised persistent solver (vadv_ws.h); four waves per
+// workgroup, group sequence blockIdx, blockIdx + gridDim, ...
+template <typename T, typename C, bool NT, bool FAST>
+__global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_kernel(Components comps,
+                                                               const T *__restrict__ wcon,
+                                                               PipeArgs a) {
+    extern __shared__ unsigned char lds_raw[];
+    constexpr int AUX = NT ? 2 : 0;
+    const int64_t first = blockIdx.x;
+    if (first >= a.ngroups) return;
+    const WsSeq q{first, int64_t(gridDim.x), (a.ngroups - first + gridDim.x - 1) / gridDim.x};
+    C *cbuf = reinterpret_cast<C *>(lds_raw) + (threadIdx.x & 63);
+    C *dbuf = cbuf + ws_rows(a.nz) * 64;
+    for (int c = 0; c < a.ncomp; ++c) {
+        const Component &cp = comps.c[c];
+        const Rsrc R{make_rsrc(cp.stage, a.span_bytes), make_rsrc(cp.pos, a.span_bytes),
+                     make_rsrc(cp.tens, a.span_bytes), make_rsrc(cp.tensstage, a.span_bytes),
+                     make_rsrc(wcon, a.span_bytes)};
+        WsSolver<T, C, AUX, FAST>::run(R, cp.wshift, a, q, cbuf, dbuf);
+    }
+}
+
 int ensure_scratch(Context *ctx, void **scratch, size_t *have, size_t bytes) {
     if (*have >= bytes) return 0;
     if (*scratch) {
@@ -406,6 +429,33 @@ hipError_t launch_pipe(int depth, bool nt, unsigned grid, size_t lds_bytes, cons
               : launch_pipe_k<T, C, 8, false>(grid, lds_bytes, comps, wcon, pa, ctx);
 }
 
+template <typename T, typename C, bool NT, bool FAST>
+hipError_t launch_ws_k(int per_cu_limit, uint64_t blocks, size_t lds_bytes, const Components &comps,
+                       const T *wcon, const PipeArgs &pa, Context *ctx) {
+    const void *fn = reinterpret_cast<const void *>(&vadv_ws_kernel<T, C, NT, FAST>);
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kWsWaves, lds_bytes);
+    if (e != hipSuccess) return e;
+    per_cu = std::max(per_cu, 1);
+    if (per_cu_limit > 0) per_cu = std::min(per_cu, per_cu_limit);
+    const uint64_t resident = uint64_t(ctx->compute_units) * uint64_t(per_cu);
+    const unsigned grid = unsigned(blocks < resident ? blocks : resident);
+    hipLaunchKernelGGL((vadv_ws_kernel<T, C, NT, FAST>), dim3(grid), dim3(64 * kWsWaves), lds_bytes,
+                       ctx->stream, comps, wcon, pa);
+    return hipGetLastError();
+}
+
+template <typename T, typename C>
+hipError_t launch_ws(bool nt, bool fast, int per_cu_limit, uint64_t blocks, size_t lds_bytes,
+                     const Components &comps, const T *wcon, const PipeArgs &pa, Context *ctx) {
+    if (fast)
+        return nt ? launch_ws_k<T, C, true, true>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx)
+                  : launch_ws_k<T, C, false, true>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx);
+    return nt ? launch_ws_k<T, C, true, false>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx)
+              : launch_ws_k<T, C, false, false>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx);
+}
+
 template <int P>
 int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_launch *l,
                 double *seconds) {
@@ -448,11 +498,23 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
                               (g->nz - 1 + g->hk) * g->sk;
     const bool buffer_ok = g->si > 0 && g->sj > 0 && g->sk > 0 &&
                            uint64_t(last_elem + 1) * sizeof(T) < (uint64_t(1) << 32);
-    if (variant == 0) variant = lds_fits ? (buffer_ok ? 3 : 1) : 2;
-    SB_REQUIRE(variant >= 1 && variant <= 3, "vadv variant %d not available", variant);
-    SB_REQUIRE(variant == 2 || lds_fits,
+    const size_t ws_lds = ws_lds_bytes<C>(g->nz);
+    const bool ws_fits = ws_lds <= ctx->lds_per_block;
+    // default: the warp-specialised solver where the single-wave pipeline is
+    // held to one group per CU by LDS (nz >= 96) and every workgroup gets at
+    // least two groups; otherwise the pipelined solver (measured on MI355X:
+    // 512x512x160 FP64 1.10x, float32 storage 1.75x; 256x256x80 0.88x)
+    const bool ws_default = ws_fits && buffer_ok &&
+                            uint64_t(last_elem + 1 + g->sk) * sizeof(T) < kWsDropOffset &&
+                            g->nz >= 96 && blocks >= 2 * uint64_t(ctx->compute_units);
+    if (variant == 0) variant = ws_default ? 4 : lds_fits ? (buffer_ok ? 3 : 1) : 2;
+    SB_REQUIRE(variant >= 1 && variant <= 4, "vadv variant %d not available", variant);
+    SB_REQUIRE(variant == 2 || variant == 4 || lds_fits,
                "nz = %lld needs %zu bytes of LDS per workgroup, device allows %zu",
                (long long)g->nz, lds_bytes + kStaticLds, ctx->lds_per_block);
+    SB_REQUIRE(variant != 4 || ws_fits,
+               "nz = %lld needs %zu bytes of LDS per workgroup, device allows %zu",
+               (long long)g->nz, ws_lds, ctx->lds_per_block);
 
     CD<C> *gscratch = nullptr;
     if (variant == 2) {
@@ -470,11 +532,27 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
     const int depth = (l && l->unroll) ? l->unroll : (g->nz >= 128 ? 8 : 4);
     SB_REQUIRE(depth == 4 || depth == 8 || depth == 16, "vadv prefetch depth %d not in {4, 8, 16}",
                depth);
-    // launch->bz: bit 0 non-temporal streamed loads, bit 1 helper wave (variant 1);
-    // 0 selects the default, -1 forces plain loads
+    // launch->bz: bit 0 non-temporal streamed loads, bit 1 helper wave (variant 1),
+    // bit 2 reciprocal estimate + Newton steps instead of division (variant 4,
+    // float32 storage with FP64 arithmetic only); 0 selects the default, -1
+    // forces plain loads
     int flags = l ? l->bz : 0;
-    if (flags == 0) flags = (6.0 * field_bytes < 512e6) ? 1 : 0;
+    if (flags == 0) flags = (variant != 4 && 6.0 * field_bytes < 512e6) ? 1 : 0;
     if (flags < 0) flags = 0;
+    const bool fast_div = (flags & 4) != 0;
+    SB_REQUIRE(!fast_div || (variant == 4 && P == SB_F32_F64),
+               "the fast reciprocal needs variant 4 and float32 storage with FP64 arithmetic");
+    if (variant == 4) {
+        SB_REQUIRE(buffer_ok && uint64_t(last_elem + 1 + g->sk) * sizeof(T) < kWsDropOffset,
+                   "warp-specialised vadv needs positive strides and fields below 4 GiB");
+        PipeArgs pa{a.nx, a.ny, a.nz, a.si, a.sj, a.sk, a.tiles_i, a.ncomp, int64_t(blocks),
+                    uint32_t(uint64_t(last_elem + 1) * sizeof(T)), 0};
+        const int per_cu_limit = l ? l->bx : 0;
+        return timed_launch(l, seconds, [&]() -> hipError_t {
+            return launch_ws<T, C>(flags & 1, fast_div, per_cu_limit, blocks, ws_lds, comps, wcon,
+                                   pa, ctx);
+        });
+    }
     if (variant == 3) {
         // byte span from a field's interior origin to its last addressable
         // element (halo included); buffer offsets are 32-bit

@@ -72,14 +72,20 @@ def _check(subs, hosts, dtype):
         assert np.array_equal(sub.download("utensstage")[sl], hf["utensstage"][sl]), sub.domain
 
 
+# columns up to the LDS limit of the FP64 solvers (nz = 160)
+TALL_DOMAINS = [(100, 20, 160), (64, 8, 133), (40, 40, 50), (3, 70, 97)]
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32_f64"])
-def test_batch_steps_match_oracle(gpu, prec):
-    from stencil_benchmarks_amd import batch
+@pytest.mark.parametrize("variant", [3, 4])
+@pytest.mark.parametrize("domains", [DOMAINS, TALL_DOMAINS], ids=["short", "tall"])
+def test_batch_steps_match_oracle(gpu, prec, variant, domains):
+    from stencil_benchmarks_amd import batch, native
 
     dtype = np.float64 if prec == "f64" else np.float32
-    subs, hosts = _make_batch(dtype)
+    subs, hosts = _make_batch(dtype, domains=domains)
     for _ in range(3):
-        batch.step(subs, compute=np.float64)
+        batch.step(subs, compute=np.float64, vadv_options=native.launch_options(variant=variant))
         for sub, hf in zip(subs, hosts):
             oracle_step(hf, sub.domain, sub.halo, dtype)
     _check(subs, hosts, dtype)

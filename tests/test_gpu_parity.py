@@ -159,7 +159,7 @@ def _vadv_expected(fields, domain, halo, prec, components):
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4])
 @pytest.mark.parametrize("all_components", [False, True])
 @pytest.mark.parametrize("domain", VADV_DOMAINS)
 def test_vadv(gpu, prec, variant, all_components, domain):
@@ -190,6 +190,55 @@ def test_vadv_pipelined_many_groups(gpu, prec, domain, depth):
                      extra=dict(unroll=depth, bz=1))
     expected = _expected(lambda *f: oracle.vadv(*f, domain, halo), fields[:5], prec)
     assert np.array_equal(got[inner(domain, halo)], expected)
+
+
+# nz around the warp-specialised solver's block (4 levels) and prefetch ring
+# (16 levels), up to the FP64 LDS limit of 160 levels; more column groups
+# than resident workgroups
+WS_DOMAINS = [(128, 700, 19), (70, 530, 9), (64, 1300, 2), (65, 90, 17), (130, 40, 20),
+              (96, 3, 41), (200, 60, 160), (33, 17, 5)]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("domain", WS_DOMAINS)
+@pytest.mark.parametrize("nt", [1, -1])
+def test_vadv_warp_specialised(gpu, prec, domain, nt):
+    halo = (1, 0, 0)
+    dtype = _dtype(prec)
+    fields = make_fields(domain, halo, dtype, count=8, seed=sum(domain))
+    got, = _run_vadv(gpu, fields, [3], domain, halo, _prec(gpu, prec), False, 4,
+                     extra=dict(bz=nt))
+    expected = _expected(lambda *f: oracle.vadv(*f, domain, halo), fields[:5], prec)
+    assert np.array_equal(got[inner(domain, halo)], expected)
+
+
+# Tolerance of the fast reciprocal (float32 storage, FP64 arithmetic): the
+# result differs from the FP64 oracle on the float32 inputs by at most
+# 1e-6 relative to max(|oracle|, 1e-3), well inside the 1e-5 bound of the
+# float32 form.
+FAST_RTOL = 1e-6
+
+
+@pytest.mark.parametrize("domain", [(200, 60, 160), (70, 37, 9), (65, 90, 17), (256, 64, 80)])
+def test_vadv_fast_reciprocal(gpu, domain):
+    halo = (1, 0, 0)
+    fields = make_fields(domain, halo, np.float32, count=8, seed=5 + sum(domain))
+    got, = _run_vadv(gpu, fields, [3], domain, halo, gpu.SB_F32_F64, False, 4,
+                     extra=dict(bz=4 | 1))
+    expected = _expected(lambda *f: oracle.vadv(*f, domain, halo), fields[:5], "f32_f64")
+    got = got[inner(domain, halo)].astype(np.float64)
+    ref = expected.astype(np.float64)
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-3)
+    assert err.max() <= FAST_RTOL, err.max()
+
+
+def test_vadv_fast_reciprocal_needs_float32_storage(gpu):
+    from stencil_benchmarks_amd.benchmark import ExecutionError
+
+    domain, halo = (64, 2, 8), (1, 0, 0)
+    fields = make_fields(domain, halo, np.float64, count=8, seed=1)
+    with pytest.raises(ExecutionError, match="fast reciprocal"):
+        _run_vadv(gpu, fields, [3], domain, halo, gpu.SB_F64, False, 4, extra=dict(bz=4))
 
 
 @pytest.mark.parametrize("flags", [1, 2, 3])
