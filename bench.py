@@ -204,6 +204,36 @@ def bench_batched(compute_dtype, storage_dtype, repeats=2):
             "steps": 10, "points_per_step": points // 10}
 
 
+def bench_plain_steps(steps=10):
+    """The batched workload with the plain FP64 kernels: per subdomain and step
+    one sb_hdiff and one sb_vadv launch; kernel time only (sum of per-launch
+    event times, so launch gaps do not count against the plain path)."""
+    from stencil_benchmarks_amd import batch
+
+    specs = batch.random_specs(96, 32, 512, 40, 160, seed=0)
+    subs = [batch.Subdomain(s.domain, s.halo, np.float64, seed=s.seed) for s in specs]
+    sec = ctypes.c_double()
+    total = 0.0
+    for sub in subs:
+        f = sub.fields
+        for _ in range(steps):
+            cur, nxt = f[f"u{sub.parity}"], f[f"u{sub.parity ^ 1}"]
+            native.call("sb_hdiff", native.SB_F64, ctypes.c_void_p(cur.pointer()),
+                        ctypes.c_void_p(f["coeff"].pointer()), ctypes.c_void_p(nxt.pointer()),
+                        sub.geom, native.launch_options(), ctypes.byref(sec))
+            total += sec.value
+            ptrs = [nxt, f["upos"], f["utens"], f["utensstage"], f["wcon"]]
+            arr = (ctypes.c_void_p * 8)(*([p.pointer() for p in ptrs] + [0, 0, 0]))
+            native.call("sb_vadv", native.SB_F64, 0, arr, sub.geom, native.launch_options(),
+                        ctypes.byref(sec))
+            total += sec.value
+            sub.advance(1)
+    nbytes = steps * sum(s.compulsory_bytes_per_step() for s in subs)
+    for s in subs:
+        s.free()
+    return {"seconds": total, "gbs": nbytes / total / 1e9}
+
+
 def bench_serving(count=160, load=0.8):
     """Poisson stream of hdiff+vadv requests, float32 storage, FP64 arithmetic."""
     from stencil_benchmarks_amd import serve
@@ -267,6 +297,16 @@ def cpu_baseline(headline, budget_s=12.0):
     }
 
 
+def profiled_traffic(key):
+    """HBM traffic per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs
+    (scaled by the copy-kernel calibration), committed in profiles/traffic.json."""
+    path = Path(__file__).resolve().parent / "profiles" / "traffic.json"
+    try:
+        return json.loads(path.read_text())[key]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
@@ -289,6 +329,8 @@ def main():
     if args.only and args.only != "hdiff":
         run_single(args)
         return
+    if args.only == "hdiff":
+        args.quick = True
     log("hdiff 512x512x160 float64 (headline)")
     head = bench_hdiff_headline(args.steps, args.warmup)
     mean_s = float(head["times"].mean())
@@ -323,7 +365,11 @@ def main():
         extras["batched_f64"] = b64
         extras["batched_f32s_f64c"] = b32
         extras["batched_f32_speedup_vs_f64"] = b64["seconds"] / b32["seconds"]
-        log(f"batched f64 {b64['gbs']:.0f} GB/s, f32 {b32['gbs']:.0f} GB/s")
+        plain = bench_plain_steps()
+        extras["plain_f64_same_workload"] = plain
+        extras["batched_f32_speedup_vs_plain_f64"] = plain["seconds"] / b32["seconds"]
+        log(f"batched f64 {b64['gbs']:.0f} GB/s, f32 {b32['gbs']:.0f} GB/s, plain f64 "
+            f"{plain['gbs']:.0f} GB/s: f32 batched {extras['batched_f32_speedup_vs_plain_f64']:.2f}x")
         log("serving: Poisson stream at 80% of capacity")
         extras["serving"] = bench_serving()
         log(f"serving p50 {extras['serving']['p50_latency_ms']:.2f} ms "
@@ -355,7 +401,7 @@ def main():
             "halo": [3, 3, 3],
             "kernel": "sb_hdiff variant 1 (LDS tile 64x16)",
             "compulsory_bytes_per_launch": head["bytes"],
-            "device": info["name"],
+            "device": info["name"] or f"{info['arch']} ({info['compute_units']} CUs)",
         },
         "roofline": {
             "bound": "hbm",
@@ -363,7 +409,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic("hdiff_512x512x160_f64"),
         },
         "cpu_baseline": baseline,
         "kernel_mean_us": mean_s * 1e6,
@@ -378,10 +424,13 @@ def run_single(args):
     if args.only == "vadv":
         out["vadv_512x512x160_f64_gbs"], out["mean_s"] = bench_vadv(
             (512, 512, 160), np.float64, native.SB_F64, args.steps)
+        out["vadv_512x512x160_f32s_f64c_gbs"], out["mean_s_f32s"] = bench_vadv(
+            (512, 512, 160), np.float32, native.SB_F32_F64, args.steps)
     elif args.only == "stream":
         out["stream_gbs"] = bench_stream(args.steps)
     elif args.only == "batched":
         out["batched_f64"] = bench_batched(np.float64, np.float64)
+        out["batched_f32s_f64c"] = bench_batched(np.float64, np.float32)
     elif args.only == "serving":
         out["serving"] = bench_serving(count=60)
     print(json.dumps(out))

@@ -171,11 +171,9 @@ __global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_batch_kernel(
         const int64_t m = d.vstride;
         const int64_t r = item - prefix[s];
         const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, d.tiles_i, 1, groups, d.span_bytes, 0};
-        const Rsrc R{make_rsrc(d.out, d.span_bytes), make_rsrc(d.pos, d.span_bytes),
-                     make_rsrc(d.tens, d.span_bytes), make_rsrc(d.ts, d.span_bytes),
-                     make_rsrc(d.wcon, d.span_bytes)};
+        const WsFields f{d.out, d.pos, d.tens, d.ts, d.wcon, d.span_bytes};
         // u component: wcon neighbour at i+1
-        WsSolver<T, C, 0, FAST>::run(R, d.si, a, WsSeq{r, m, (groups - r + m - 1) / m}, lane0,
+        WsSolver<T, C, 0, FAST>::run(f, d.si, a, WsSeq{r, m, (groups - r + m - 1) / m}, lane0,
                                      lane0 + ws_rows(d.nz) * 64);
     }
 }

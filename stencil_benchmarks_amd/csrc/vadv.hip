@@ -348,10 +348,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_kernel(Components comps
     C *dbuf = cbuf + ws_rows(a.nz) * 64;
     for (int c = 0; c < a.ncomp; ++c) {
         const Component &cp = comps.c[c];
-        const Rsrc R{make_rsrc(cp.stage, a.span_bytes), make_rsrc(cp.pos, a.span_bytes),
-                     make_rsrc(cp.tens, a.span_bytes), make_rsrc(cp.tensstage, a.span_bytes),
-                     make_rsrc(wcon, a.span_bytes)};
-        WsSolver<T, C, AUX, FAST>::run(R, cp.wshift, a, q, cbuf, dbuf);
+        const WsFields f{cp.stage, cp.pos, cp.tens, cp.tensstage, wcon, a.span_bytes};
+        WsSolver<T, C, AUX, FAST>::run(f, cp.wshift, a, q, cbuf, dbuf);
     }
 }
 

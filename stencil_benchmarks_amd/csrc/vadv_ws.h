@@ -102,6 +102,22 @@ struct WsSeq {
     int64_t first, stride, count;
 };
 
+// Field pointers (interior origins) of one component; each role builds the
+// buffer resources of the fields it touches itself, after branching on its
+// role, which keeps the other roles' descriptors out of its SGPRs.
+struct WsFields {
+    const void *stage, *pos, *tens;
+    void *ts;
+    const void *wcon;
+    uint32_t span_bytes;
+};
+
+__device__ __forceinline__ Rsrc ws_rsrc(const WsFields &f) {
+    return Rsrc{make_rsrc(f.stage, f.span_bytes), make_rsrc(f.pos, f.span_bytes),
+                make_rsrc(f.tens, f.span_bytes), make_rsrc(f.ts, f.span_bytes),
+                make_rsrc(f.wcon, f.span_bytes)};
+}
+
 template <typename T, typename C, int AUX, bool FAST>
 struct WsSolver {
     static constexpr int M = kWsBlock, R = kWsRing, U = kWsRing / kWsBlock;
@@ -184,10 +200,15 @@ struct WsSolver {
                 const C *db = dbuf + rb;
                 C out[M];
                 if (b >= 1 && b * M + M <= c.nz) {
+                    C dv[M], cv[M];  // block's slot reads first (latency overlaps)
 #pragma unroll
                     for (int j = 0; j < M; ++j) {
-                        const int o = in_block<!GODD>(j);
-                        x = db[o] - cb[o] * x;
+                        dv[j] = db[in_block<!GODD>(j)];
+                        cv[j] = cb[in_block<!GODD>(j)];
+                    }
+#pragma unroll
+                    for (int j = 0; j < M; ++j) {
+                        x = dv[j] - cv[j] * x;
                         out[j] = dtr * (x - C(s.ring[u * M + j]));
                     }
                 } else {
@@ -219,8 +240,9 @@ struct WsSolver {
     // Group loops run in pairs (even-slot group, odd-slot group) so a ring
     // flows through straight-line code: a join between the two slot
     // orientations would copy registers with loads in flight and drain them.
-    __device__ __forceinline__ static void backward(const Rsrc &Rs, const PipeArgs &a,
+    __device__ __forceinline__ static void backward(const WsFields &f, const PipeArgs &a,
                                                     const WsSeq &q, C *cbuf, C *dbuf) {
+        const Rsrc Rs = ws_rsrc(f);
         const int nz = int(a.nz), nb = (nz + M - 1) / M;
         BState s;
         // iteration 0: no backward group; load group 0's ring at step nb
@@ -299,8 +321,9 @@ struct WsSolver {
 
     // P1 works at steps 1 .. nb of an iteration and loads the next group's
     // ring in its idle steps nb+1, nb+2.
-    __device__ __forceinline__ static void prep_d0(const Rsrc &Rs, const PipeArgs &a,
+    __device__ __forceinline__ static void prep_d0(const WsFields &f, const PipeArgs &a,
                                                    const WsSeq &q, C *dbuf) {
+        const Rsrc Rs = ws_rsrc(f);
         const int nz = int(a.nz), nb = (nz + M - 1) / M;
         P1State s;
         Col c = column(a, group(q, 0), 0);
@@ -363,6 +386,9 @@ struct WsSolver {
                 C *cb = cbuf + rb;
                 C *db = dbuf + rb;
                 if (b >= 1 && b * M + M <= c.nz - 1) {
+                    C d0[M];  // block's slot reads first (latency overlaps)
+#pragma unroll
+                    for (int j = 0; j < M; ++j) d0[j] = db[in_block<ODD>(j)];
 #pragma unroll
                     for (int j = 0; j < M; ++j) {
                         const int r = u * M + j, o = in_block<ODD>(j);
@@ -370,7 +396,7 @@ struct WsSolver {
                         const C s_n = C(s.rwn[r]) + C(s.rw[r]);
                         const C cs = (C(0.25) * s_n) * bet_m;
                         const C corr = cs_p * (st_m - st_c) - cs * (st_n - st_c);
-                        db[o] = db[o] + corr;  // e = d0 + corr
+                        db[o] = d0[j] + corr;  // e = d0 + corr
                         cb[o] = s_n;
                         st_m = st_c;
                         st_c = st_n;
@@ -411,8 +437,10 @@ struct WsSolver {
     }
 
     // P2 works at steps 2 .. nb+1 and loads the next ring in its idle step nb+2.
-    __device__ __forceinline__ static void prep_e(const Rsrc &Rs, const PipeArgs &a, int64_t wshift,
-                                                  const WsSeq &q, C *cbuf, C *dbuf) {
+    __device__ __forceinline__ static void prep_e(const WsFields &f, const PipeArgs &a,
+                                                  int64_t wshift, const WsSeq &q, C *cbuf,
+                                                  C *dbuf) {
+        const Rsrc Rs = ws_rsrc(f);
         const int nz = int(a.nz), nb = (nz + M - 1) / M;
         P2State s;
         Col c = column(a, group(q, 0), wshift);
@@ -538,15 +566,15 @@ struct WsSolver {
     }
 
     // All four waves of the workgroup call this with the same arguments.
-    __device__ __forceinline__ static void run(const Rsrc &Rs, int64_t wshift, const PipeArgs &a,
+    __device__ __forceinline__ static void run(const WsFields &f, int64_t wshift, const PipeArgs &a,
                                                const WsSeq &q, C *cbuf, C *dbuf) {
         const int role = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
         if (role == 0)
-            backward(Rs, a, q, cbuf, dbuf);
+            backward(f, a, q, cbuf, dbuf);
         else if (role == 1)
-            prep_d0(Rs, a, q, dbuf);
+            prep_d0(f, a, q, dbuf);
         else if (role == 2)
-            prep_e(Rs, a, wshift, q, cbuf, dbuf);
+            prep_e(f, a, wshift, q, cbuf, dbuf);
         else
             chain(a, q, cbuf, dbuf);
     }
