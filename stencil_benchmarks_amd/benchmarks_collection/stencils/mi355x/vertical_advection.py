@@ -1,11 +1,14 @@
 """Vertical advection on the MI355X.
 
 One lane per (i, j) column with contiguous i, forward sweep and fused
-backward substitution in one kernel (csrc/vadv.hip).  ``Fused`` keeps the
-Thomas coefficients in LDS when nz fits (nz <= 160 in FP64 arithmetic) and
-in a global FP64 scratch otherwise; ``LdsColumn`` and ``GlobalScratch`` pin
-one storage.  Arithmetic follows stencils/base.py:415-473 (bit-exact in
-FP64; with float32 storage the default arithmetic is FP64 throughout).
+backward substitution in one kernel (csrc/vadv.hip).  ``Fused`` picks the
+solver: the warp-specialised one (four waves per 64-column group) for
+nz >= 96, the single-wave pipeline below that, both with the Thomas
+coefficients in LDS (nz <= 160 in FP64 arithmetic), and a global FP64
+scratch beyond.  ``LdsColumn``, ``GlobalScratch``, ``Pipelined`` and
+``WarpSpecialised`` pin one solver.  Arithmetic follows
+stencils/base.py:415-473 (bit-exact in FP64; with float32 storage the
+default arithmetic is FP64 throughout).
 """
 
 import ctypes
@@ -56,3 +59,11 @@ class LdsColumn(VerticalAdvectionMixin, VerticalAdvectionStencil):
 
 class GlobalScratch(VerticalAdvectionMixin, VerticalAdvectionStencil):
     VARIANT = 2
+
+
+class Pipelined(VerticalAdvectionMixin, VerticalAdvectionStencil):
+    VARIANT = 3
+
+
+class WarpSpecialised(VerticalAdvectionMixin, VerticalAdvectionStencil):
+    VARIANT = 4

@@ -1,3 +1,5 @@
+import shutil
+import subprocess
 import sys
 from pathlib import Path
 
@@ -11,6 +13,13 @@ if str(ROOT) not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device 0)")
+    # the CPU oracle (test infrastructure) is built here when missing, so the
+    # CPU suite does not depend on a prior build step; on the GPU box it is
+    # prebuilt and travels with the tree
+    oracle_lib = ROOT / "oracle" / "liboracle.so"
+    if not oracle_lib.exists() and shutil.which("gcc"):
+        subprocess.run(["make", "-C", str(ROOT / "oracle"), "liboracle.so"], check=False,
+                       capture_output=True)
 
 
 @pytest.fixture(scope="session")
