@@ -18,8 +18,9 @@ from .mixin import StencilMixin
 
 class HorizontalDiffusionMixin(StencilMixin):
     block_size = Parameter(
-        "tile size: i (64), j (8, 16, 32), k-planes per workgroup (1, 2, 4, 8, 16)",
-        (64, 16, 8),
+        "tile size: i (64), j (8, 16, 32), k-planes per workgroup (1, 2, 4, 8, 16; "
+        "0: measured default, 2 for FP64 and 8 otherwise)",
+        (64, 16, 0),
     )
 
     VARIANT = 1
@@ -27,7 +28,7 @@ class HorizontalDiffusionMixin(StencilMixin):
     def setup(self):
         super().setup()
         bx, by, bz = self.block_size
-        if self.VARIANT == 1 and (bx != 64 or by not in (8, 16, 32) or bz not in (1, 2, 4, 8, 16)):
+        if self.VARIANT == 1 and (bx != 64 or by not in (8, 16, 32) or bz not in (0, 1, 2, 4, 8, 16)):
             raise ParameterError(f"unsupported tile size {self.block_size}")
 
     @property
