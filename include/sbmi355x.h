@@ -134,8 +134,11 @@ int sb_laplacian(int prec, int along_x, int along_y, int along_z,
  * Replaces kernel(double*, inp, coeff, out) of the hdiff variants
  * (stencils/cuda_hip/horizontal_diffusion.py:50-128 and
  * templates/horizontal_diffusion_*.j2); semantics of base.py:276-311.
- * Requires hi, hj >= 2.  Variants: 0 default, 1 LDS-staged tile,
- * 2 register j-scan with cross-lane exchange. */
+ * Requires hi, hj >= 2.  Variants: 0 default (= 1), 1 LDS-staged tile
+ * (launch->by = tile rows 8/16/32, ->bz = k-planes per workgroup 1..16, 0
+ * measured default; ->unroll bit 0 non-temporal out stores, bit 1
+ * non-temporal coeff loads), 3 per-point recomputation through the caches
+ * (the reference's OnTheFly schedule). */
 int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
              const sb_geom *g, const sb_launch *l, double *seconds);
 
@@ -145,10 +148,17 @@ int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
  * templates/vertical_advection_*.j2); semantics of base.py:349-501.
  * `fields` holds the device pointers in the reference's argument order
  * (base.py:329-337): 8 entries, or 16 when all_components != 0.  Only the
- * *tensstage fields are written; ccol/dcol are used as scratch by
- * variant 2 only; datacol is never touched.  Requires hi >= 1 (all
- * components: hj >= 1 too).  Variants: 0 default, 1 Thomas coefficients
- * held in LDS, 2 coefficients in the ccol/dcol fields. */
+ * *tensstage fields are written; ccol, dcol and datacol are never touched
+ * (the Thomas coefficients stay on chip, or in a library-owned scratch for
+ * variant 2).  Requires hi >= 1 (all components: hj >= 1 too).  Variants:
+ * 0 default (4 for nz >= 96 with at least two column groups per resident
+ * workgroup, else 3; 2 beyond the LDS limit of nz = 160), 1 one wave per
+ * group with the coefficients in LDS, 2 coefficients in a global FP64
+ * scratch, 3 single-wave pipeline fusing the forward sweep of one group with
+ * the backward of the previous, 4 warp-specialised solver (four waves share
+ * one LDS slot buffer).  launch->bz: bit 0 non-temporal streamed loads,
+ * bit 2 (variant 4, SB_F32_F64 only) reciprocal estimate plus Newton steps
+ * instead of the correctly rounded division; -1 plain loads. */
 int sb_vadv(int prec, int all_components, void *const *fields,
             const sb_geom *g, const sb_launch *l, double *seconds);
 
@@ -180,8 +190,11 @@ int sb_stream_check(int prec, const void *x, size_t n, double expected,
  * positive strides, every field below 4 GiB, nz small enough that a 64-column
  * Thomas buffer fits in LDS (nz <= 160 with FP64 arithmetic).
  * Launch options: hdiff_launch->by / ->bz / ->unroll = tile height / planes
- * per item / non-temporal flags (as sb_hdiff); vadv_launch->unroll = prefetch
- * depth (4 or 8), ->bz = 1 for non-temporal streamed loads. */
+ * per item / non-temporal flags (as sb_hdiff); vadv_launch->variant = 4
+ * (default: warp-specialised solver, items assigned round-robin) or 3
+ * (single-wave pipeline, items from a device counter; ->unroll = prefetch
+ * depth 4 or 8, ->bz = 1 non-temporal streamed loads); with variant 4,
+ * ->bz bit 2 selects the fast reciprocal (SB_F32_F64 only). */
 typedef struct sb_subdomain {
     void *u[2];
     void *coeff;
@@ -191,9 +204,9 @@ typedef struct sb_subdomain {
     int32_t reserved;
 } sb_subdomain;
 
-/* One timestep for `count` subdomains (two persistent launches drawing
- * tiles and column groups from a device work queue).  Does not flip the
- * parities; the caller does. */
+/* One timestep for `count` subdomains (two persistent launches: hdiff tiles
+ * from a device work queue, vadv column groups per subdomain).  Does not
+ * flip the parities; the caller does. */
 int sb_batch_step(int prec, const sb_subdomain *subs, int count, const sb_launch *hdiff_launch,
                   const sb_launch *vadv_launch, double *seconds);
 /* A fixed set of subdomains advanced `steps` steps per run: device tables
