@@ -136,8 +136,9 @@ int sb_laplacian(int prec, int along_x, int along_y, int along_z,
  * templates/horizontal_diffusion_*.j2); semantics of base.py:276-311.
  * Requires hi, hj >= 2.  Variants: 0 default (= 1), 1 LDS-staged tile
  * (launch->by = tile rows 8/16/32, ->bz = k-planes per workgroup 1..16, 0
- * measured default; ->unroll bit 0 non-temporal out stores, bit 1
- * non-temporal coeff loads), 3 per-point recomputation through the caches
+ * measured default; ->unroll 0 = default (non-temporal coeff loads), else
+ * 4 | flags with bit 0 non-temporal out stores, bit 1 non-temporal coeff
+ * loads), 3 per-point recomputation through the caches
  * (the reference's OnTheFly schedule). */
 int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
              const sb_geom *g, const sb_launch *l, double *seconds);

@@ -6,14 +6,19 @@
 // the centre and the off-axis neighbours are V-wide aligned vector loads
 // (one dwordx4 for V*sizeof(T) == 16), the two along-axis neighbours that
 // fall outside the vector are scalar loads, the rest are register shuffles.
-// A workgroup sweeps ROWS rows (points with equal axis-1/axis-2 index) so
-// every thread keeps several rows' loads in flight.
+// A workgroup sweeps R consecutive rows (points with equal axis-1/axis-2
+// index; launch->by = 4, 8 or 16, default 4) so every thread keeps several
+// rows' loads in flight, and row blocks are remapped so that consecutive
+// blocks run on the same XCD: the off-axis neighbour rows a block shares with
+// the next one then hit that XCD's L2 instead of being fetched again.
 //
 // Arithmetic follows the verification oracles of stencils/base.py:
 //   onesided  (inp[x+e] + inp[x]) / 2                       (base.py:204)
 //   symmetric (inp[x+e] + inp[x-e]) / 2                     (base.py:221)
 //   laplacian r = 0; r += (2 inp - inp[x+e]) - inp[x-e] per enabled axis in
 //             x, y, z order                                 (base.py:243-253)
+#include <type_traits>
+
 #include "sb_internal.h"
 
 namespace sb {
@@ -27,10 +32,9 @@ struct BasicArgs {
     int32_t axis;       // permuted averaging axis
     int32_t lap_mask;   // bit o: Laplacian along ORIGINAL axis o
     int32_t perm[3];    // permuted index of original axis o
-    int32_t pad;
+    int32_t xcds;       // XCDs of the device (row-block remap)
 };
 
-constexpr int ROWS = 4;
 constexpr int BLOCK = 256;
 
 template <typename T, int V>
@@ -136,16 +140,17 @@ __device__ __forceinline__ void vector_points(const T *__restrict__ inp, T *__re
     *reinterpret_cast<VT *>(out + p) = res;
 }
 
-template <typename T, typename C, int OP, int V>
+template <typename T, typename C, int OP, int V, int ROWS>
 __global__ __launch_bounds__(BLOCK) void basic_kernel(const T *__restrict__ inp,
                                                       T *__restrict__ out, BasicArgs a) {
     const int64_t i = (int64_t(blockIdx.y) * BLOCK + threadIdx.x) * V;
     if (i >= a.n[0]) return;
     const int64_t rows = a.n[1] * a.n[2];
     const bool full = (i + V <= a.n[0]);
+    const uint32_t rb = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-        const int64_t row = int64_t(blockIdx.x) * ROWS + r;
+        const int64_t row = int64_t(rb) * ROWS + r;
         if (row >= rows) break;
         const int64_t j = row % a.n[1];
         const int64_t k = row / a.n[1];
@@ -206,21 +211,30 @@ int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis
     Context *ctx = context();
     if (!ctx) return 1;
     const int64_t rows = a.n[1] * a.n[2];
-    const dim3 grid(unsigned((rows + ROWS - 1) / ROWS),
+    const int R = (l && l->by) ? l->by : 4;
+    SB_REQUIRE(R == 4 || R == 8 || R == 16, "rows per workgroup %d not in {4, 8, 16}", R);
+    a.xcds = ctx->xcds;
+    const dim3 grid(unsigned((rows + R - 1) / R),
                     unsigned((a.n[0] + int64_t(BLOCK) * V - 1) / (int64_t(BLOCK) * V)));
+    auto launch_v = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, ctx->stream, inp, out, a);
+    };
+    auto launch_r = [&](auto v_tag) {
+        constexpr int VV = decltype(v_tag)::value;
+        if (R == 4) launch_v(basic_kernel<T, C, OP, VV, 4>);
+        else if (R == 16) launch_v(basic_kernel<T, C, OP, VV, 16>);
+        else launch_v(basic_kernel<T, C, OP, VV, 8>);
+    };
     return timed_launch(l, seconds, [&]() -> hipError_t {
         switch (V) {
             case 4:
-                hipLaunchKernelGGL((basic_kernel<T, C, OP, (sizeof(T) == 4 ? 4 : 1)>), grid,
-                                   dim3(BLOCK), 0, ctx->stream, inp, out, a);
+                launch_r(std::integral_constant<int, (sizeof(T) == 4 ? 4 : 1)>{});
                 break;
             case 2:
-                hipLaunchKernelGGL((basic_kernel<T, C, OP, 2>), grid, dim3(BLOCK), 0,
-                                   ctx->stream, inp, out, a);
+                launch_r(std::integral_constant<int, 2>{});
                 break;
             default:
-                hipLaunchKernelGGL((basic_kernel<T, C, OP, 1>), grid, dim3(BLOCK), 0,
-                                   ctx->stream, inp, out, a);
+                launch_r(std::integral_constant<int, 1>{});
                 break;
         }
         return hipGetLastError();

@@ -156,8 +156,10 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
     // defaults measured on MI355X (512 x 512 x 160): 64 x 16 tiles, 2 planes
     // per workgroup in FP64 arithmetic, 8 with FP32 arithmetic
     const int kc = (l && l->bz) ? l->bz : (sizeof(C) == 8 && sizeof(T) == 8 ? 2 : 8);
-    // launch->unroll: bit 0 non-temporal out stores, bit 1 non-temporal coeff loads
-    const int flags = l ? (l->unroll & 3) : 0;
+    // launch->unroll: 0 = default (non-temporal coeff loads, measured best on
+    // MI355X), else 4 | flags with bit 0 non-temporal out stores and bit 1
+    // non-temporal coeff loads
+    const int flags = (l && l->unroll) ? (l->unroll & 3) : 2;
     SB_REQUIRE(variant == 0 || variant == 1 || variant == 3, "hdiff variant %d not available",
                variant);
     if (variant == 3) {

@@ -12,16 +12,23 @@ thread_local std::string g_error;
 std::mutex g_ctx_mutex;
 std::vector<Context> g_contexts;
 
-// Touch every 16-byte chunk of the flush buffer (read-modify-write keeps the
-// compiler from dropping it and pulls lines through L2 and the MALL).
+// Read every 16-byte chunk of the flush buffer: the lines displace the
+// caller's data from L2 and the Infinity Cache, and, unlike a
+// read-modify-write sweep, leave no dirty lines whose write-back would compete
+// with the next (timed) kernel for HBM.  The buffer holds zeros, so the
+// conditional store never happens; it keeps the loads alive.
 __global__ __launch_bounds__(256) void flush_kernel(uint4 *buf, size_t n) {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const size_t stride = size_t(gridDim.x) * blockDim.x;
+    uint4 acc = make_uint4(0u, 0u, 0u, 0u);
     for (; i < n; i += stride) {
-        uint4 v = buf[i];
-        v.x += 1u;
-        buf[i] = v;
+        const uint4 v = buf[i];  // plain load: the line must allocate in the caches
+        acc.x ^= v.x;
+        acc.y ^= v.y;
+        acc.z ^= v.z;
+        acc.w ^= v.w;
     }
+    if ((acc.x & acc.y & acc.z & acc.w) == 0xFFFFFFFFu) buf[0] = acc;
 }
 
 int init_context(Context &c, int device) {
