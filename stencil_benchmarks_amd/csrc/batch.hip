@@ -20,8 +20,11 @@
 //     warp-specialised column solver of the plain kernel (vadv_ws.h; the
 //     single-wave pipeline of vadv_column.h with variant 3) with an LDS
 //     buffer sized for the tallest column in the batch.  The warp-specialised
-//     kernel takes items round-robin (item i on workgroup i mod grid): its
-//     LDS is full at nz = 160, which leaves no room to broadcast a ticket.  Workgroups that
+//     kernel takes no tickets (its LDS is full at nz = 160, which leaves no
+//     room to broadcast one): the host deals its items to the workgroups
+//     ahead of time (snake order over subdomains sorted by item cost, from a
+//     cost model of the solver's pipeline, (groups + 1) x (levels / 4 + 3)
+//     barrier steps per item), and every workgroup walks its own list.  Workgroups that
 //     take consecutive items of one subdomain walk adjacent groups level by
 //     level, so the waves of the chip touch the same few k planes (and
 //     pages) at a time, as in the plain kernel; runs of consecutive groups
@@ -43,6 +46,16 @@ namespace sb {
 namespace {
 
 constexpr int kMinGroupsPerItem = 8;  // amortises the pipeline's fill and drain
+// Groups per item of the warp-specialised solver: its fill and drain cost
+// nb + 3 steps per item; with the host's cost-ordered schedule, larger items
+// do not unbalance the workgroups (16 measured best on the 96-subdomain
+// batch, 8 and 64 slower).
+constexpr int kWsGroupsPerItem = 16;
+
+// One scheduled item of the warp-specialised vadv: subdomain rank, residue.
+struct VItem {
+    int32_t rank, residue;
+};
 
 // Device descriptor of one subdomain for one step parity.
 struct SubDev {
@@ -137,12 +150,11 @@ void vadv_batch_kernel(const SubDev *__restrict__ subs, const int64_t *__restric
         if (item >= total) break;
         const int s = find_sub(prefix, count, item);
         const SubDev d = subs[s];
-        const int64_t groups = int64_t(d.tiles_i) * d.ny;
+        const int64_t groups = vadv_groups(d.nx, d.ny);
         const int64_t m = d.vstride;
         const int64_t g0 = item - prefix[s];
         const int64_t n = (groups - g0 + m - 1) / m;  // groups g0, g0 + m, ...
-        const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, d.tiles_i, 1, groups,
-                         d.span_bytes, 0};
+        const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, 1, groups, d.span_bytes, 0};
         const Rsrc R{make_rsrc(d.out, d.span_bytes), make_rsrc(d.pos, d.span_bytes),
                      make_rsrc(d.tens, d.span_bytes), make_rsrc(d.ts, d.span_bytes),
                      make_rsrc(d.wcon, d.span_bytes)};
@@ -158,19 +170,28 @@ void vadv_batch_kernel(const SubDev *__restrict__ subs, const int64_t *__restric
     }
 }
 
+// Workgroup w walks items list[off[w]] .. list[off[w+1]-1] (the host's
+// schedule); w is the logical worker of this workgroup, numbered so that
+// consecutive workers share an XCD: the schedule hands consecutive items of
+// a subdomain (adjacent groups of a row, which share the cache lines at
+// their edges) to consecutive workers.
 template <typename T, typename C, bool FAST>
 __global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_batch_kernel(
-    const SubDev *__restrict__ subs, const int64_t *__restrict__ prefix, int count) {
+    const SubDev *__restrict__ subs, const int32_t *__restrict__ off,
+    const VItem *__restrict__ list, int xcds) {
     extern __shared__ unsigned char lds_raw[];
     C *lane0 = reinterpret_cast<C *>(lds_raw) + (threadIdx.x & 63);
-    const int64_t total = prefix[count];
-    for (int64_t item = blockIdx.x; item < total; item += gridDim.x) {
-        const int s = find_sub(prefix, count, item);
+    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x, uint32_t(xcds));
+    const int32_t end = off[w + 1];
+    for (int32_t i = off[w]; i < end; ++i) {
+        const VItem it = list[i];
+        const int s = int(__builtin_amdgcn_readfirstlane(it.rank));
         const SubDev d = subs[s];
-        const int64_t groups = int64_t(d.tiles_i) * d.ny;
+        const int64_t groups = vadv_groups(d.nx, d.ny);
         const int64_t m = d.vstride;
-        const int64_t r = item - prefix[s];
-        const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, d.tiles_i, 1, groups, d.span_bytes, 0};
+        const int64_t r = int64_t(__builtin_amdgcn_readfirstlane(it.residue));
+        const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, 1, groups, d.span_bytes,
+                         uint32_t(xcds)};
         const WsFields f{d.out, d.pos, d.tens, d.ts, d.wcon, d.span_bytes};
         // u component: wcon neighbour at i+1
         WsSolver<T, C, 0, FAST>::run(f, d.si, a, WsSeq{r, m, (groups - r + m - 1) / m}, lane0,
@@ -192,11 +213,15 @@ struct Plan {
     SubDev *subs[2] = {nullptr, nullptr};
     int64_t *hprefix = nullptr, *vprefix = nullptr;
     unsigned long long *counters = nullptr;
+    int32_t *voff = nullptr;  // warp-specialised vadv schedule (CSR per worker)
+    VItem *vlist = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // host copies
     std::vector<SubDev> host_subs[2];
     std::vector<int64_t> host_hprefix, host_vprefix;
+    std::vector<int32_t> host_voff;
+    std::vector<VItem> host_vlist;
 };
 
 int hdiff_ty(const sb_launch *l) { return (l && l->by) ? l->by : 16; }
@@ -211,6 +236,8 @@ int hdiff_flags(const sb_launch *l) { return l && l->unroll ? (l->unroll & 3) : 
 // streamed loads); for variant 4, bz bit 2 selects the fast reciprocal
 // (float32 storage with FP64 arithmetic only)
 int vadv_variant(const sb_launch *l) { return (l && l->variant) ? l->variant : 4; }
+// groups per item of the warp-specialised solver (launch->by, 0 = default)
+int vadv_ws_per_item(const sb_launch *l) { return (l && l->by > 0) ? l->by : kWsGroupsPerItem; }
 int vadv_pf(const sb_launch *l) { return (l && l->unroll) ? l->unroll : 8; }
 int vadv_nt(const sb_launch *l) { return (l && l->bz) ? (l->bz > 0 ? 1 : 0) : 0; }
 bool vadv_fast(const sb_launch *l) { return l && l->bz > 0 && (l->bz & 4); }
@@ -258,6 +285,47 @@ unsigned vadv_workers(const Plan &p, const Context *ctx) {
     return unsigned(fit * ctx->compute_units);
 }
 
+// Schedule the warp-specialised vadv items onto `workers` workgroups.  Cost
+// of an item of n groups of nb blocks: (n + 1) (nb + skew) barrier steps
+// (the solver's pipeline with its fill and drain).  Subdomains are taken in
+// order of decreasing item cost, the items of one subdomain in residue
+// order, and dealt to the workers in a snake: 0 .. W-1, then W-1 .. 0, and
+// so on.  This balances the workers nearly as well as longest-first onto
+// the least-loaded worker, and keeps consecutive residues of a subdomain
+// (adjacent groups of a row) on adjacent workers, which share an XCD.
+void schedule_ws(Plan &p, int workers) {
+    std::vector<int64_t> item_cost(size_t(p.count));
+    for (int r = 0; r < p.count; ++r) {
+        const SubDev &d = p.host_subs[0][r];
+        const int64_t groups = vadv_groups(d.nx, d.ny);
+        const int64_t nb = (d.nz + kWsBlock - 1) / kWsBlock;
+        item_cost[size_t(r)] = ((groups + d.vstride - 1) / d.vstride + 1) * (nb + kWsSkew);
+    }
+    std::vector<int> ranks(size_t(p.count));
+    std::iota(ranks.begin(), ranks.end(), 0);
+    std::stable_sort(ranks.begin(), ranks.end(),
+                     [&](int a, int b) { return item_cost[size_t(a)] > item_cost[size_t(b)]; });
+    std::vector<int32_t> owner;
+    std::vector<VItem> items;
+    items.reserve(size_t(p.host_vprefix[p.count]));
+    std::vector<int32_t> per(size_t(workers), 0);
+    int64_t i = 0;
+    for (int r : ranks) {
+        for (int32_t res = 0; res < p.host_subs[0][r].vstride; ++res, ++i) {
+            const int64_t pass = i / workers, o = i % workers;
+            const int32_t w = int32_t((pass & 1) ? workers - 1 - o : o);
+            items.push_back(VItem{int32_t(r), res});
+            owner.push_back(w);
+            ++per[size_t(w)];
+        }
+    }
+    p.host_voff.assign(size_t(workers) + 1, 0);
+    for (int w = 0; w < workers; ++w) p.host_voff[w + 1] = p.host_voff[w] + per[w];
+    p.host_vlist.resize(items.size());
+    std::vector<int32_t> fill(p.host_voff.begin(), p.host_voff.end() - 1);
+    for (size_t k = 0; k < items.size(); ++k) p.host_vlist[size_t(fill[owner[k]]++)] = items[k];
+}
+
 // Build descriptors and prefix sums (largest subdomains first).
 void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch *hl, Plan &p,
                   const Context *ctx) {
@@ -273,6 +341,8 @@ void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch
     p.max_nz = 2;
     for (int i = 0; i < count; ++i) p.max_nz = std::max<int>(p.max_nz, int(subs[i].geom.nz));
     const int64_t workers = vadv_workers(p, ctx);
+    const bool ws = vadv_variant(&p.vl) == 4;
+    const int64_t per_item = ws ? vadv_ws_per_item(&p.vl) : kMinGroupsPerItem;
     for (int flip = 0; flip < 2; ++flip) p.host_subs[flip].resize(count);
     p.host_hprefix.assign(count + 1, 0);
     p.host_vprefix.assign(count + 1, 0);
@@ -282,8 +352,8 @@ void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch
         const int64_t last = (g.nx - 1 + g.hi) * g.si + (g.ny - 1 + g.hj) * g.sj +
                              (g.nz - 1 + g.hk) * g.sk;
         const int64_t tiles_i = (g.nx + 63) / 64;
-        const int64_t groups = tiles_i * g.ny;
-        const int64_t vitems = std::max<int64_t>(1, std::min(workers, groups / kMinGroupsPerItem));
+        const int64_t groups = vadv_groups(g.nx, g.ny);
+        const int64_t vitems = std::max<int64_t>(1, std::min(workers, groups / per_item));
         for (int flip = 0; flip < 2; ++flip) {
             const int cur = (s.parity + flip) & 1;
             SubDev d{};
@@ -310,6 +380,9 @@ void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch
         p.host_hprefix[r + 1] = p.host_hprefix[r] + hitems;
         p.host_vprefix[r + 1] = p.host_vprefix[r] + vitems;
     }
+    p.host_voff.clear();
+    p.host_vlist.clear();
+    if (ws) schedule_ws(p, int(workers));
 }
 
 // Enqueue one step (both kernels) on ctx->stream.
@@ -341,12 +414,14 @@ hipError_t enqueue_step(const Plan &p, int flip, unsigned long long *counters, C
     // vadv
     const unsigned vgrid = vadv_workers(p, ctx);
     if (vadv_variant(&p.vl) == 4) {
+        // the schedule was built for this grid: one list per workgroup
+        if (p.host_voff.size() != size_t(vgrid) + 1) return hipErrorInvalidValue;
         const size_t wlds = ws_lds_bytes<C>(p.max_nz);
         auto launch_w = [&](auto kernel) -> hipError_t {
             (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(wlds));
             hipLaunchKernelGGL(kernel, dim3(vgrid), dim3(64 * kWsWaves), wlds, ctx->stream, subs,
-                               p.vprefix, p.count);
+                               p.voff, p.vlist, ctx->xcds);
             return hipGetLastError();
         };
         if constexpr (sizeof(T) == 4 && sizeof(C) == 8)
@@ -385,7 +460,10 @@ int upload_tables(Plan &p, Context *ctx) {
     const size_t subs_bytes = sizeof(SubDev) * std::max(1, p.count);
     const size_t pre_bytes = sizeof(int64_t) * (p.count + 1);
     const size_t ctr_bytes = sizeof(unsigned long long) * 2 * std::max(1, p.steps);
-    const size_t need = 2 * subs_bytes + 2 * pre_bytes + ctr_bytes + 256;
+    const size_t off_bytes = sizeof(int32_t) * p.host_voff.size();
+    const size_t list_bytes = sizeof(VItem) * p.host_vlist.size();
+    const size_t sched = 2 * subs_bytes + 2 * pre_bytes + ctr_bytes;  // 8-byte aligned
+    const size_t need = sched + off_bytes + 8 + list_bytes + 256;
     if (p.dev_bytes < need) {
         if (p.dev) SB_HIP(hipFree(p.dev));
         p.dev = nullptr;
@@ -398,6 +476,14 @@ int upload_tables(Plan &p, Context *ctx) {
     p.hprefix = reinterpret_cast<int64_t *>(base + 2 * subs_bytes);
     p.vprefix = reinterpret_cast<int64_t *>(base + 2 * subs_bytes + pre_bytes);
     p.counters = reinterpret_cast<unsigned long long *>(base + 2 * subs_bytes + 2 * pre_bytes);
+    p.voff = reinterpret_cast<int32_t *>(base + sched);
+    p.vlist = reinterpret_cast<VItem *>(base + ((sched + off_bytes + 7) & ~size_t(7)));
+    if (off_bytes)
+        SB_HIP(hipMemcpyAsync(p.voff, p.host_voff.data(), off_bytes, hipMemcpyHostToDevice,
+                              ctx->stream));
+    if (list_bytes)
+        SB_HIP(hipMemcpyAsync(p.vlist, p.host_vlist.data(), list_bytes, hipMemcpyHostToDevice,
+                              ctx->stream));
     if (p.count) {
         SB_HIP(hipMemcpyAsync(p.subs[0], p.host_subs[0].data(), sizeof(SubDev) * p.count,
                               hipMemcpyHostToDevice, ctx->stream));

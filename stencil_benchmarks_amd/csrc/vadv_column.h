@@ -64,11 +64,10 @@ __device__ __forceinline__ T stream_load(const T *p) {
 // ---------------------------------------------------------------------------
 struct PipeArgs {
     int64_t nx, ny, nz, si, sj, sk;
-    int32_t tiles_i;
     int32_t ncomp;
-    int64_t ngroups;
+    int64_t ngroups;  // column groups: ceil(nx * ny / 64)
     uint32_t span_bytes;  // addressable bytes from each field's interior origin
-    uint32_t pad;
+    uint32_t xcds;        // XCDs of the device: consecutive groups share one XCD's L2
 };
 
 // Buffer-resource addressing: every field is one descriptor based at its
@@ -108,23 +107,37 @@ struct FwdState {
     C st_m, st_c, w_c, wn_c, c_prev, d_prev;
 };
 
-// Column-group geometry: lane byte offsets and the group's row byte offset.
+// Column-group geometry: lane byte offsets of the group's columns relative
+// to a uniform row byte offset (the row of the group's first column), which
+// level offsets add as the buffer instructions' scalar offset.
 struct GroupOff {
     uint32_t lane, lane_wn, row;
 };
 
+// Number of column groups of an nx x ny domain.
+__host__ __device__ constexpr int64_t vadv_groups(int64_t nx, int64_t ny) {
+    return (nx * ny + 63) / 64;
+}
+
+// Group g holds columns 64 g .. 64 g + 63 of the domain's nx * ny columns in
+// i-fastest order: a group runs on into the next row where nx is not a
+// multiple of 64, so only the domain's last group can have idle lanes.
+// Idle lanes duplicate the last column: same inputs, same outputs, so their
+// (identical) stores are harmless.  Column indices fit 32 bits (every field
+// is below 4 GiB).
 template <typename T>
 __device__ __forceinline__ GroupOff group_offsets(const PipeArgs &a, int64_t g, int64_t wshift) {
-    const int64_t tile_i = g % a.tiles_i;
-    const int64_t j = g / a.tiles_i;
-    const int64_t i = tile_i * 64 + (threadIdx.x & 63);
-    // idle lanes past nx duplicate column nx-1: same inputs, same outputs,
-    // so their (identical) stores are harmless
-    const int64_t ic = i < a.nx ? i : a.nx - 1;
+    const uint32_t nx = uint32_t(a.nx), cols = uint32_t(a.nx * a.ny);
+    const uint32_t first = min(uint32_t(g) * 64u, cols - 1);
+    const uint32_t j0 = first / nx;  // uniform: the group's first row
+    uint32_t col = first + (threadIdx.x & 63);
+    col = col < cols ? col : cols - 1;
+    const uint32_t j = col / nx, i = col - j * nx;
+    const int64_t e = int64_t(i) * a.si + int64_t(j - j0) * a.sj;
     GroupOff o;
-    o.lane = uint32_t(ic * a.si * int64_t(sizeof(T)));
-    o.lane_wn = uint32_t((ic * a.si + wshift) * int64_t(sizeof(T)));
-    o.row = uint32_t(j * a.sj * int64_t(sizeof(T)));
+    o.lane = uint32_t(e * int64_t(sizeof(T)));
+    o.lane_wn = uint32_t((e + wshift) * int64_t(sizeof(T)));
+    o.row = uint32_t(int64_t(j0) * a.sj * int64_t(sizeof(T)));
     return o;
 }
 
