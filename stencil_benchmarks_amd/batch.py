@@ -51,7 +51,7 @@ class Subdomain:
     """Seven device fields of one subdomain plus its step parity."""
 
     def __init__(self, domain, halo=(3, 3, 3), dtype="float64", alignment=64, seed=0,
-                 layout=(2, 1, 0), host_fields=None, fill=True):
+                 layout=(2, 1, 0), host_fields=None, fill=True, arena=None):
         self.domain = tuple(int(d) for d in domain)
         self.halo = tuple(int(h) for h in halo)
         self.dtype = np.dtype(dtype)
@@ -59,7 +59,13 @@ class Subdomain:
         self.template = array.alloc_array(shape, self.dtype, layout, alignment, self.halo)
         self.strides = tuple(s // self.dtype.itemsize for s in self.template.strides)
         self.geom = geometry(self.domain, self.strides, self.halo)
-        self.fields = {name: DeviceField(self.template, self.halo) for name in FIELDS}
+        self.fields = {}
+        try:
+            for name in FIELDS:  # from `arena` if given (a DeviceArena), else own allocations
+                self.fields[name] = DeviceField(self.template, self.halo, arena=arena)
+        except BaseException:
+            self.free()
+            raise
         self.parity = 0
         self.steps_done = 0
         if host_fields is not None:

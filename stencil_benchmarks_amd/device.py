@@ -10,6 +10,7 @@ allocation offset changes) and lets the kernels use aligned dwordx4 loads
 whenever the row and plane pitches allow it.
 """
 
+import bisect
 import ctypes
 import weakref
 
@@ -42,6 +43,66 @@ def _free(ptr):
         pass
 
 
+class DeviceArena:
+    """One device allocation carved into blocks: first fit over a free list
+    sorted by offset, neighbours coalesced on release.
+
+    For callers that admit and retire many fields (the request server):
+    taking a block is host bookkeeping only, with no ``hipMalloc`` and no
+    ``hipFree`` (which waits for the device) per field.  Blocks are
+    `alignment`-aligned.  A block must not be released while kernels that
+    use it are in flight.
+    """
+
+    def __init__(self, nbytes: int, alignment: int = DEVICE_ALIGNMENT, buffer=None):
+        self.alignment = int(alignment)
+        self.capacity = int(nbytes) // self.alignment * self.alignment
+        # `buffer`: an existing allocation of at least nbytes (ptr, free())
+        self.buffer = DeviceBuffer(self.capacity) if buffer is None else buffer
+        self._free = [(0, self.capacity)]  # (offset, size), sorted by offset
+        self.in_use = 0
+
+    def alloc(self, nbytes: int) -> "ArenaBlock":
+        size = max(self.alignment, -(-int(nbytes) // self.alignment) * self.alignment)
+        for index, (offset, free) in enumerate(self._free):
+            if free >= size:
+                if free == size:
+                    del self._free[index]
+                else:
+                    self._free[index] = (offset + size, free - size)
+                self.in_use += size
+                return ArenaBlock(self, offset, size)
+        raise MemoryError(f"device arena: no free block of {size} bytes "
+                          f"({self.capacity - self.in_use} of {self.capacity} free, fragmented)")
+
+    def _release(self, offset: int, size: int):
+        self.in_use -= size
+        index = bisect.bisect_left(self._free, (offset, 0))
+        if index < len(self._free) and self._free[index][0] == offset + size:  # merge right
+            size += self._free[index][1]
+            del self._free[index]
+        if index > 0 and sum(self._free[index - 1]) == offset:  # merge left
+            offset, size = self._free[index - 1][0], self._free[index - 1][1] + size
+            del self._free[index - 1]
+            index -= 1
+        self._free.insert(index, (offset, size))
+
+    def close(self):
+        self.buffer.free()
+
+
+class ArenaBlock:
+    """A block of a DeviceArena (the DeviceBuffer interface: ptr, nbytes, free)."""
+
+    def __init__(self, arena: DeviceArena, offset: int, size: int):
+        self.ptr = arena.buffer.ptr + offset
+        self.nbytes = size
+        self._finalizer = weakref.finalize(self, arena._release, offset, size)
+
+    def free(self):
+        self._finalizer()
+
+
 def _first_element_offset(host: np.ndarray) -> int:
     """Bytes from the lowest addressed element to element (0, ..., 0)."""
     return int(sum(-(n - 1) * s for n, s in zip(host.shape, host.strides) if s < 0))
@@ -50,7 +111,7 @@ def _first_element_offset(host: np.ndarray) -> int:
 class DeviceField:
     """Device mirror of a (possibly padded) host field."""
 
-    def __init__(self, host: np.ndarray, index_to_align=None):
+    def __init__(self, host: np.ndarray, index_to_align=None, arena: DeviceArena = None):
         self.shape = host.shape
         self.dtype = host.dtype
         self.strides = host.strides
@@ -58,7 +119,8 @@ class DeviceField:
         index = tuple(index_to_align) if index_to_align is not None else (0,) * host.ndim
         self.origin = array_tools.origin_offset(host, index)
         pad = (-self.origin) % DEVICE_ALIGNMENT
-        self.buffer = DeviceBuffer(self.span + pad + DEVICE_ALIGNMENT)
+        nbytes = self.span + pad + DEVICE_ALIGNMENT
+        self.buffer = arena.alloc(nbytes) if arena is not None else DeviceBuffer(nbytes)
         base = self.buffer.ptr
         self.base = base + ((-(base + self.origin)) % DEVICE_ALIGNMENT)
         self.index_to_align = index

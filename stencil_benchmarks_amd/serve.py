@@ -7,7 +7,10 @@ its U[0,1) fields.  The server
 * admits requests in arrival order once they have arrived and their
   fields fit in the free-HBM budget (head-of-line: a request that does not
   fit blocks later ones, so large requests are not starved);
-* keeps every admitted request's fields resident in HBM until it finishes;
+* keeps every admitted request's fields resident in HBM until it finishes,
+  in a device arena allocated once per run, before its clock starts
+  (admission and retirement are host bookkeeping: no allocation or release
+  of device memory, whose release waits for the device, per request);
 * advances all active requests together, one timestep per tick, in one
   batched launch pair (``sb_batch_step``): requests that arrive while
   others are on the GPU join at the next tick and share the persistent
@@ -28,6 +31,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import batch, native
+from .device import DeviceArena
 
 
 @dataclass
@@ -67,13 +71,14 @@ def poisson_requests(rate, count, seed=0, nmin=16, nmax=1024, nz_choices=(40, 80
 
 
 def footprint_bytes(domain, halo, dtype, alignment=64):
-    """Device bytes of a request's seven fields (padded like alloc_array)."""
+    """Device bytes of a request's seven fields (padded like alloc_array,
+    plus the 256-byte alignment of each field's interior origin)."""
     from .tools import array
 
     shape = tuple(d + 2 * h for d, h in zip(domain, halo))
     strides, span = array.compute_strides(shape, np.dtype(dtype).itemsize, (2, 1, 0), alignment)
     last = sum((n - 1) * s for n, s in zip(shape, strides)) + np.dtype(dtype).itemsize
-    return 7 * (last + 2 * 256)
+    return 7 * (last + 3 * 256)
 
 
 @dataclass
@@ -108,17 +113,16 @@ class Server:
         self.dtype = np.dtype(dtype)
         self.compute = np.dtype(compute)
         self.halo = tuple(halo)
-        if hbm_budget is None:
-            free, _ = native.mem_info()
-            hbm_budget = int(free * 0.6)
-        self.budget = int(hbm_budget)
+        self.budget = None if hbm_budget is None else int(hbm_budget)  # None: 60% of free HBM
+        self.arena = None
         self.hdiff_options = hdiff_options
         self.vadv_options = vadv_options
         self.keep_results = keep_results
         self.results = {}
 
     def _admit(self, req, now):
-        req.sub = batch.Subdomain(req.domain, self.halo, self.dtype, seed=req.seed)
+        req.sub = batch.Subdomain(req.domain, self.halo, self.dtype, seed=req.seed,
+                                  arena=self.arena)
         req.admitted = now
         req.remaining = req.steps
 
@@ -137,9 +141,22 @@ class Server:
         With realtime=False every request is available at t=0 (closed loop,
         measures capacity).
         """
+        budget = self.budget
+        if budget is None:
+            free, _ = native.mem_info()
+            budget = int(free * 0.6)
+        # no larger than every request resident at once needs
+        need = sum(footprint_bytes(r.domain, self.halo, self.dtype) for r in requests)
+        self.arena = DeviceArena(max(1, min(budget, need)))
+        try:
+            return self._serve(requests, realtime)
+        finally:
+            self.arena.close()
+            self.arena = None
+
+    def _serve(self, requests, realtime):
         pending = deque(sorted(requests, key=lambda r: r.arrival))
         active, done = [], []
-        used = 0
         ticks = 0
         bytes_moved = 0.0
         point_steps = 0.0
@@ -148,14 +165,16 @@ class Server:
         while pending or active:
             now = time.perf_counter() - start
             while pending and (not realtime or pending[0].arrival <= now):
-                need = footprint_bytes(pending[0].domain, self.halo, self.dtype)
-                if used + need > self.budget and active:
-                    break  # wait for memory (head of line)
-                req = pending.popleft()
+                req = pending[0]
+                try:
+                    self._admit(req, now)
+                except MemoryError:
+                    if active:
+                        break  # wait for memory (head of line)
+                    raise
+                pending.popleft()
                 if not realtime:
                     req.arrival = 0.0
-                self._admit(req, now)
-                used += need
                 active.append(req)
             if not active:
                 # idle until the next arrival
@@ -177,7 +196,6 @@ class Server:
             still = []
             for r in active:
                 if r.remaining == 0:
-                    used -= footprint_bytes(r.domain, self.halo, self.dtype)
                     self._retire(r, now)
                     done.append(r)
                 else:
@@ -192,16 +210,19 @@ def run_plain(requests, dtype="float64", halo=(3, 3, 3)):
     """Serve requests one at a time with the plain (unbatched) kernels.
 
     Each step is one ``sb_hdiff`` and one ``sb_vadv`` launch on the
-    request's own fields; returns (wall seconds, bytes, point-steps).
+    request's own fields, which come from a device arena allocated before
+    the clock starts (as the server's); returns (wall seconds, bytes,
+    point-steps).
     """
     dtype = np.dtype(dtype)
     prec = native.precision_code(dtype)
     total_bytes = 0.0
     point_steps = 0.0
+    arena = DeviceArena(max(footprint_bytes(r.domain, halo, dtype) for r in requests))
     native.call("sb_synchronize")
     start = time.perf_counter()
     for req in requests:
-        sub = batch.Subdomain(req.domain, halo, dtype, seed=req.seed)
+        sub = batch.Subdomain(req.domain, halo, dtype, seed=req.seed, arena=arena)
         f = sub.fields
         for _ in range(req.steps):
             cur, nxt = f[f"u{sub.parity}"], f[f"u{sub.parity ^ 1}"]
@@ -216,4 +237,6 @@ def run_plain(requests, dtype="float64", halo=(3, 3, 3)):
             point_steps += sub.points
         native.call("sb_synchronize")
         sub.free()
-    return time.perf_counter() - start, total_bytes, point_steps
+    elapsed = time.perf_counter() - start
+    arena.close()
+    return elapsed, total_bytes, point_steps
