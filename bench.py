@@ -264,6 +264,16 @@ def bench_serving(count=160, load=0.8):
     return out
 
 
+def bench_host_stream(threads, array_size=1 << 27):
+    """McCalpin STREAM on the host cores (3 x 1 GiB float64 arrays,
+    non-temporal stores), GB/s per kernel: the memory ceiling of the CPU
+    baseline, which runs on the same threads."""
+    from stencil_benchmarks_amd.benchmarks_collection.stream import mc_calpin
+
+    rows = mc_calpin.Native(array_size=array_size, ntimes=6, threads=threads).run()
+    return {r["name"]: r["bandwidth"] / 1e3 for r in rows}
+
+
 def cpu_baseline(headline, budget_s=12.0):
     """OpenMP port of the reference CPU hdiff on the headline fields."""
     from oracle import oracle
@@ -383,6 +393,9 @@ def main():
     if not args.no_cpu:
         log("CPU baseline (OpenMP port) on the headline fields")
         baseline = cpu_baseline(head)
+        log("host STREAM (the CPU baseline's memory ceiling)")
+        baseline["host_stream_gbs"] = bench_host_stream(baseline["cores"])
+        baseline["frac_of_host_triad"] = baseline["value"] / baseline["host_stream_gbs"]["triad"]
 
     record = {
         "metric": METRIC,

@@ -1,5 +1,5 @@
-"""STREAM benchmarks: the MI355X GPU STREAM."""
+"""STREAM benchmarks: the MI355X GPU STREAM and the host (CPU) STREAM."""
 
-from . import mi355x
+from . import mc_calpin, mi355x
 
-__all__ = ["mi355x"]
+__all__ = ["mc_calpin", "mi355x"]

@@ -202,9 +202,13 @@ def build(commands):
     return root
 
 
-def main(*args, **kwargs):
+def command_group():
+    """The click group of every registered benchmark (the sbench command)."""
     from . import benchmarks_collection  # noqa: F401  (registers the benchmarks)
 
     ordered = sorted(benchmark.REGISTRY, key=lambda b: (b.__module__, b.__name__))
-    group = build([(command_path(b), _command(b)) for b in ordered])
-    return group(*args, **kwargs, obj=types.SimpleNamespace())
+    return build([(command_path(b), _command(b)) for b in ordered])
+
+
+def main(*args, **kwargs):
+    return command_group()(*args, **kwargs, obj=types.SimpleNamespace())
