@@ -116,7 +116,13 @@ int sb_flush_caches(void);
  * Computations of the generated basic kernels
  * (stencils/cuda_hip/basic.py:101-132, templates/basic_1d.j2, basic_3d.j2),
  * interior points only, with the arithmetic order of the verification
- * oracles (stencils/base.py:180-254). */
+ * oracles (stencils/base.py:180-254).  Launch options (averages and
+ * Laplacian): ->variant 0 default, 1 row kernel (->by = rows per workgroup
+ * 4/8/16), 2 marching kernel (a thread walks ->bz = 8/16/32/64 rows of the
+ * second-smallest-stride axis keeping three rows in registers; needs
+ * 16-byte vector access along the unit-stride axis and no term along the
+ * largest-stride axis).  The default marches where a term lies along the
+ * second axis. */
 int sb_copy(int prec, const void *inp, void *out, const sb_geom *g,
             const sb_launch *l, double *seconds);
 /* out = (inp[x + e_axis] + inp[x]) / 2 (base.py:191-205). */

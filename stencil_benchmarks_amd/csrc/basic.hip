@@ -165,6 +165,112 @@ __global__ __launch_bounds__(BLOCK) void basic_kernel(const T *__restrict__ inp,
     }
 }
 
+// Marching variant for stencils whose terms lie along permuted axes 0 and 1
+// only: a thread owns V consecutive points of axis 0 and walks JB rows of
+// axis 1 (one plane of axis 2), keeping rows j-1, j, j+1 in registers, so
+// every input row is loaded once per thread (plus the two halo rows of its
+// run) instead of once per neighbouring output row; the two axis-0
+// neighbours outside the vector are scalar loads of the adjacent lanes'
+// lines.  Terms are summed in original-axis order, as the row kernel does.
+template <typename T, typename C, int OP, int V, int JB>
+__global__ __launch_bounds__(BLOCK) void march_kernel(const T *__restrict__ inp,
+                                                      T *__restrict__ out, BasicArgs a) {
+    using VT = Vec<T, V>;
+    const int64_t i = (int64_t(blockIdx.y) * BLOCK + threadIdx.x) * V;
+    if (i >= a.n[0]) return;
+    const int64_t chunks = (a.n[1] + JB - 1) / JB;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
+    const int64_t j0 = int64_t(b % uint32_t(chunks)) * JB;
+    const int64_t k = int64_t(b / uint32_t(chunks));
+    const int64_t jn = a.n[1] - j0 < JB ? a.n[1] - j0 : JB;
+    const int64_t s1 = a.s[1];
+    int64_t p = i * a.s[0] + j0 * s1 + k * a.s[2];
+    if (i + V > a.n[0]) {  // ragged end of axis 0: scalar points
+        const int64_t last = a.n[0] - i;
+        for (int64_t jj = 0; jj < jn; ++jj, p += s1)
+            for (int64_t e = 0; e < last; ++e)
+                out[p + e * a.s[0]] = T(scalar_point<T, C, OP, V>(inp, p + e * a.s[0], a));
+        return;
+    }
+    // axis-1 neighbours needed: below (j-1) and above (j+1)
+    bool below = false, above = false;
+    if constexpr (OP == OP_LAP) {
+        below = above = false;
+#pragma unroll
+        for (int o = 0; o < 3; ++o)
+            if ((a.lap_mask & (1 << o)) && a.perm[o] == 1) below = above = true;
+    } else {
+        above = a.axis == 1;
+        below = OP == OP_AVG2 && a.axis == 1;
+    }
+    VT prev{}, cur, next{};
+    if (below) prev = *reinterpret_cast<const VT *>(inp + p - s1);
+    cur = *reinterpret_cast<const VT *>(inp + p);
+#pragma unroll 4
+    for (int64_t jj = 0; jj < jn; ++jj, p += s1) {
+        if (above) next = *reinterpret_cast<const VT *>(inp + p + s1);
+        VT res;
+        if constexpr (OP == OP_LAP) {
+            C r[V];
+#pragma unroll
+            for (int e = 0; e < V; ++e) r[e] = C(0);
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                if (!(a.lap_mask & (1 << o))) continue;
+                C plus[V], minus[V];
+                if (a.perm[o] == 0) {
+                    const C right = C(inp[p + V]);
+                    const C left = C(inp[p - 1]);
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        plus[e] = e + 1 < V ? C(cur.e[e + 1]) : right;
+                        minus[e] = e > 0 ? C(cur.e[e - 1]) : left;
+                    }
+                } else {
+#pragma unroll
+                    for (int e = 0; e < V; ++e) {
+                        plus[e] = C(next.e[e]);
+                        minus[e] = C(prev.e[e]);
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < V; ++e) r[e] += (C(2) * C(cur.e[e]) - plus[e]) - minus[e];
+            }
+#pragma unroll
+            for (int e = 0; e < V; ++e) res.e[e] = T(r[e]);
+        } else {
+            C plus[V], minus[V];
+            if (a.axis == 0) {
+                const C right = C(inp[p + V]);
+                C left = C(0);
+                if constexpr (OP == OP_AVG2) left = C(inp[p - 1]);
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    plus[e] = e + 1 < V ? C(cur.e[e + 1]) : right;
+                    minus[e] = e > 0 ? C(cur.e[e - 1]) : left;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < V; ++e) {
+                    plus[e] = C(next.e[e]);
+                    minus[e] = C(prev.e[e]);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < V; ++e) {
+                if constexpr (OP == OP_AVG1)
+                    res.e[e] = T((plus[e] + C(cur.e[e])) / C(2));
+                else
+                    res.e[e] = T((plus[e] + minus[e]) / C(2));
+            }
+        }
+        *reinterpret_cast<VT *>(out + p) = res;
+        prev = cur;
+        cur = next;
+        if (!above && jj + 1 < jn) cur = *reinterpret_cast<const VT *>(inp + p + s1);
+    }
+}
+
 template <int P, int OP>
 int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis, int lap_mask,
                       const sb_launch *l, double *seconds) {
@@ -210,12 +316,61 @@ int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis
 
     Context *ctx = context();
     if (!ctx) return 1;
+    a.xcds = ctx->xcds;
+    // launch->variant: 0 default, 1 the row kernel, 2 the marching kernel.
+    // The marching kernel needs vector access along axis 0 and no term along
+    // axis 2; by default it runs where a term lies along axis 1, whose rows
+    // it reuses (measured on MI355X, 512x512x160 FP64: Laplacian ij 3.72 ->
+    // 4.76 TB/s); with terms along axis 0 only the row kernel is faster
+    // (more workgroups in flight, nothing to reuse).
+    bool marchable = OP != OP_COPY && V > 1, axis1 = false;
+    if constexpr (OP == OP_LAP) {
+        for (int o = 0; o < 3; ++o) {
+            if ((lap_mask & (1 << o)) && a.perm[o] == 2) marchable = false;
+            if ((lap_mask & (1 << o)) && a.perm[o] == 1) axis1 = true;
+        }
+    } else {
+        if (a.axis == 2) marchable = false;
+        axis1 = a.axis == 1;
+    }
+    const int variant = l ? l->variant : 0;
+    SB_REQUIRE(variant >= 0 && variant <= 2, "basic stencil variant %d not in {0, 1, 2}", variant);
+    SB_REQUIRE(variant != 2 || marchable,
+               "the marching kernel needs vector access along the unit-stride axis and no "
+               "term along the largest-stride axis");
+    const int64_t iblocks = (a.n[0] + int64_t(BLOCK) * V - 1) / (int64_t(BLOCK) * V);
+    if (variant == 2 || (variant == 0 && marchable && axis1)) {
+        // launch->bz: rows per thread (8, 16, 32, 64; 0 = 8: about five
+        // rounds of resident workgroups at 512x512x160, a short last round
+        // costs less than with 1.25 rounds at 32 rows)
+        const int JB = (l && l->bz) ? l->bz : 8;
+        SB_REQUIRE(JB == 8 || JB == 16 || JB == 32 || JB == 64,
+                   "rows per thread %d not in {8, 16, 32, 64}", JB);
+        const int64_t chunks = (a.n[1] + JB - 1) / JB;
+        SB_REQUIRE(chunks * a.n[2] < (int64_t(1) << 31), "domain too large for one launch");
+        const dim3 grid(unsigned(chunks * a.n[2]), unsigned(iblocks));
+        auto launch_j = [&](auto v_tag) {
+            constexpr int VV = decltype(v_tag)::value;
+            auto go = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, ctx->stream, inp, out, a);
+            };
+            if (JB == 8) go(march_kernel<T, C, OP, VV, 8>);
+            else if (JB == 16) go(march_kernel<T, C, OP, VV, 16>);
+            else if (JB == 64) go(march_kernel<T, C, OP, VV, 64>);
+            else go(march_kernel<T, C, OP, VV, 32>);
+        };
+        return timed_launch(l, seconds, [&]() -> hipError_t {
+            if (V == 4 && sizeof(T) == 4)
+                launch_j(std::integral_constant<int, (sizeof(T) == 4 ? 4 : 2)>{});
+            else
+                launch_j(std::integral_constant<int, 2>{});
+            return hipGetLastError();
+        });
+    }
     const int64_t rows = a.n[1] * a.n[2];
     const int R = (l && l->by) ? l->by : 4;
     SB_REQUIRE(R == 4 || R == 8 || R == 16, "rows per workgroup %d not in {4, 8, 16}", R);
-    a.xcds = ctx->xcds;
-    const dim3 grid(unsigned((rows + R - 1) / R),
-                    unsigned((a.n[0] + int64_t(BLOCK) * V - 1) / (int64_t(BLOCK) * V)));
+    const dim3 grid(unsigned((rows + R - 1) / R), unsigned(iblocks));
     auto launch_v = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, ctx->stream, inp, out, a);
     };

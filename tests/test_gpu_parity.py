@@ -114,6 +114,51 @@ def test_basic_stencils(gpu, prec, layout, alignment, domain):
         _check_halo(got, domain, halo)
 
 
+MARCH_DOMAINS = [(130, 70, 3), (64, 33, 2), (37, 9, 4), (256, 100, 2), (2, 65, 1)]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("domain", MARCH_DOMAINS)
+@pytest.mark.parametrize("variant,rows", [(1, 0), (2, 8), (2, 32), (2, 64)])
+def test_basic_stencil_kernels(gpu, prec, domain, variant, rows):
+    """Row kernel (variant 1) and marching kernel (variant 2, `rows` per
+    thread) on vector-aligned fields: runs shorter than, equal to and
+    longer than the rows per thread, ragged ends of the unit-stride axis."""
+    halo = (1, 2, 1)
+    dtype = _dtype(prec)
+    code = _prec(gpu, prec)
+    inp, = make_fields(domain, halo, dtype, (2, 1, 0), 64, seed=sum(domain))
+    cases = []
+    for axis in (0, 1) if variant == 2 else (0, 1, 2):
+        cases.append(("sb_onesided_average", (axis,),
+                      lambda f, a=axis: oracle.onesided_average(f, a, domain, halo)))
+        cases.append(("sb_symmetric_average", (axis,),
+                      lambda f, a=axis: oracle.symmetric_average(f, a, domain, halo)))
+    masks = [(1, 1, 0), (1, 0, 0), (0, 1, 0)] + ([(1, 1, 1)] if variant == 1 else [])
+    for mask in masks:
+        cases.append(("sb_laplacian", mask,
+                      lambda f, m=mask: oracle.laplacian(f, m, domain, halo)))
+    opts = gpu.launch_options()
+    opts.variant, opts.bz = variant, rows
+    for entry, extra, func in cases:
+        out = _with_sentinel(make_fields(domain, halo, dtype, (2, 1, 0), 64)[0])
+        got, = _run(gpu, entry, [inp, out], [1], domain, halo, code, *extra, launch=opts)
+        expected = _expected(func, [inp], prec)
+        assert np.array_equal(got[inner(domain, halo)], expected), (entry, extra)
+        _check_halo(got, domain, halo)
+
+
+def test_marching_kernel_rejects_axis_2_terms(gpu):
+    from stencil_benchmarks_amd.benchmark import ExecutionError
+
+    domain, halo = (64, 8, 4), (1, 1, 1)
+    inp, out = make_fields(domain, halo, np.float64, (2, 1, 0), 64, count=2)
+    opts = gpu.launch_options()
+    opts.variant = 2
+    with pytest.raises(ExecutionError, match="marching kernel"):
+        _run(gpu, "sb_laplacian", [inp, out], [1], domain, halo, gpu.SB_F64, 1, 1, 1, launch=opts)
+
+
 HDIFF_DOMAINS = [(70, 37, 9), (64, 16, 1), (1, 1, 1), (129, 33, 3), (5, 70, 2)]
 
 
