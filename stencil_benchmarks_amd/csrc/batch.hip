@@ -257,7 +257,8 @@ int validate(int prec, const sb_subdomain *subs, int count) {
         SB_REQUIRE(g.si > 0 && g.sj > 0 && g.sk > 0, "subdomain %d: strides must be positive", i);
         const int64_t last = (g.nx - 1 + g.hi) * g.si + (g.ny - 1 + g.hj) * g.sj +
                              (g.nz - 1 + g.hk) * g.sk;
-        SB_REQUIRE(uint64_t(last + 1 + g.sk) * esize < kWsDropOffset,
+        SB_REQUIRE(uint64_t(last + 1 + g.sk) * esize < kWsDropOffset &&
+                       hdiff_tile_fits(g.nx, g.ny, g.si, g.sj, esize),
                    "subdomain %d: fields must be below 4 GiB", i);
         SB_REQUIRE(s.parity == 0 || s.parity == 1, "subdomain %d: parity must be 0 or 1", i);
     }

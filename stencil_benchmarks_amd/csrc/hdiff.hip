@@ -151,7 +151,12 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
         return 0;
     }
     HdiffArgs a{g->nx, g->ny, g->nz, g->si, g->sj, g->sk, 0, 0, ctx->xcds, 0};
-    const int variant = l ? l->variant : 0;
+    // the LDS tile (variant 1, default) addresses a plane with 32-bit offsets
+    const bool tile_fits = hdiff_tile_fits(g->nx, g->ny, g->si, g->sj, sizeof(T));
+    int variant = l ? l->variant : 0;
+    if (variant == 0) variant = tile_fits ? 1 : 3;
+    SB_REQUIRE(variant != 1 || tile_fits,
+               "the LDS-tiled hdiff needs positive i and j strides and a k-plane below 4 GiB");
     const int ty = (l && l->by) ? l->by : 16;
     // defaults measured on MI355X (512 x 512 x 160): 64 x 16 tiles, 2 planes
     // per workgroup in FP64 arithmetic, 8 with FP32 arithmetic
@@ -160,8 +165,7 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
     // MI355X), else 4 | flags with bit 0 non-temporal out stores and bit 1
     // non-temporal coeff loads
     const int flags = (l && l->unroll) ? (l->unroll & 3) : 2;
-    SB_REQUIRE(variant == 0 || variant == 1 || variant == 3, "hdiff variant %d not available",
-               variant);
+    SB_REQUIRE(variant == 1 || variant == 3, "hdiff variant %d not available", variant);
     if (variant == 3) {
         SB_REQUIRE(g->nz <= 65535, "on-the-fly hdiff supports nz <= 65535");
         const dim3 grid(unsigned((g->nx + 63) / 64), unsigned((g->ny + 3) / 4), unsigned(g->nz));

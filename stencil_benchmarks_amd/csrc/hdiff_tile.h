@@ -23,6 +23,28 @@ struct HdiffGeom {
     int64_t nx, ny, nz, si, sj, sk;
 };
 
+// Whether the tile worker's 32-bit plane offsets cover a field: positive i
+// and j strides, and point (nx + 1, ny + 1) counted from (-2, -2) below
+// 4 GiB.
+inline bool hdiff_tile_fits(int64_t nx, int64_t ny, int64_t si, int64_t sj, size_t itemsize) {
+    if (si <= 0 || sj <= 0) return false;
+    const double span = (double(nx + 3) * double(si) + double(ny + 3) * double(sj)) * double(itemsize);
+    return span < 4294967296.0;
+}
+
+// Byte offset of point (gi, gj) within a k-plane of field element type T.
+template <typename T>
+__device__ __forceinline__ uint32_t plane_offset_t(int64_t gi, int64_t gj, const HdiffGeom &a) {
+    return uint32_t((gi * a.si + gj * a.sj) * int64_t(sizeof(T)));
+}
+
+// `base` advanced by a non-negative byte offset.
+template <typename P>
+__device__ __forceinline__ P at(P base, uint32_t byte_offset) {
+    using B = __attribute__((address_space(1))) const char *;
+    return (P)((B)base + byte_offset);
+}
+
 template <typename C>
 __device__ __forceinline__ C limited(C flux, C gradient) {
     return flux * gradient > C(0) ? C(0) : flux;
@@ -74,7 +96,12 @@ struct HdiffTile {
         const int tx = threadIdx.x, ty = threadIdx.y;
         const int tid = ty * TX + tx;
         // ---- per-thread staging offsets (plane-independent) -----------------
-        int64_t off_main[MAIN];
+        // Byte offsets within a k-plane, 32-bit (a plane of a field below
+        // 4 GiB), added to the plane's uniform base: the loads take the
+        // scalar-base form with one VGPR of offset each.  Input offsets count
+        // from point (-2, -2), the lowest halo point a tile reads, so none is
+        // negative; coeff and out offsets count from (0, 0).
+        uint32_t off_main[MAIN];
         int lds_main[MAIN];
     #pragma unroll
         for (int m = 0; m < MAIN; ++m) {
@@ -82,38 +109,39 @@ struct HdiffTile {
             const int rr = r < LH ? r : LH - 1;
             const int64_t gi = min(i0 + tx, a.nx + 1);
             const int64_t gj = min(j0 + rr - 2, a.ny + 1);
-            off_main[m] = gi * a.si + gj * a.sj;
+            off_main[m] = plane_offset_t<T>(gi + 2, gj + 2, a);
             lds_main[m] = r < LH ? rr * LW + tx + 2 : -1;
         }
         const bool has_halo = tid < 4 * LH;
-        int64_t off_halo = 0;
+        uint32_t off_halo = 0;
         int lds_halo = -1;
         if (has_halo) {
             const int r = tid >> 2, q = tid & 3;
             const int c = q < 2 ? q : TX + q;  // 0, 1, TX+2, TX+3
             const int64_t gi = min(i0 + c - 2, a.nx + 1);
             const int64_t gj = min(j0 + r - 2, a.ny + 1);
-            off_halo = gi * a.si + gj * a.sj;
+            off_halo = plane_offset_t<T>(gi + 2, gj + 2, a);
             lds_halo = r * LW + c;
         }
         const int64_t gi_out = i0 + tx;
-        int64_t off_coeff[RPT];
+        // coeff (clamped into the domain) and, where in range, out
+        uint32_t off_point[RPT];
     #pragma unroll
         for (int q = 0; q < RPT; ++q) {
             const int64_t gi = min(gi_out, a.nx - 1);
             const int64_t gj = min(j0 + ty * RPT + q, a.ny - 1);
-            off_coeff[q] = gi * a.si + gj * a.sj;
+            off_point[q] = plane_offset_t<T>(gi, gj, a);
         }
 
         T pm[MAIN], ph = T(0), pc[RPT];
         auto issue = [&](int64_t k) {
-            global_ptr<const T> in_k = inp + k * a.sk;
+            global_ptr<const T> in_k = inp + (k * a.sk - 2 * a.si - 2 * a.sj);
     #pragma unroll
-            for (int m = 0; m < MAIN; ++m) pm[m] = in_k[off_main[m]];
-            if (has_halo) ph = in_k[off_halo];
+            for (int m = 0; m < MAIN; ++m) pm[m] = *at(in_k, off_main[m]);
+            if (has_halo) ph = *at(in_k, off_halo);
             global_ptr<const T> cf_k = coeff + k * a.sk;
     #pragma unroll
-            for (int q = 0; q < RPT; ++q) pc[q] = ld_nt<(FLAGS & 2) != 0>(cf_k + off_coeff[q]);
+            for (int q = 0; q < RPT; ++q) pc[q] = ld_nt<(FLAGS & 2) != 0>(at(cf_k, off_point[q]));
         };
         auto commit = [&](int buf) {
     #pragma unroll
@@ -175,11 +203,12 @@ struct HdiffTile {
                 res[q] = c0 - cf[q] * (((flx - flx_m) + fly) - fly_m);
             }
             if (gi_out < a.nx) {
-                global_ptr<T> out_k = out + k * a.sk + gi_out * a.si;
+                global_ptr<T> out_k = out + k * a.sk;
     #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
                     const int64_t gj = j0 + ty * RPT + q;
-                    if (gj < a.ny) st_nt<(FLAGS & 1) != 0>(out_k + gj * a.sj, T(res[q]));
+                    // in range, the clamped point offset is the point's own
+                    if (gj < a.ny) st_nt<(FLAGS & 1) != 0>(at(out_k, off_point[q]), T(res[q]));
                 }
             }
             if (more) commit(cur ^ 1);
