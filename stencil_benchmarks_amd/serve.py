@@ -141,6 +141,7 @@ class Server:
         With realtime=False every request is available at t=0 (closed loop,
         measures capacity).
         """
+        requests = list(requests)
         budget = self.budget
         if budget is None:
             free, _ = native.mem_info()
