@@ -140,7 +140,9 @@ int sb_laplacian(int prec, int along_x, int along_y, int along_z,
  * Replaces kernel(double*, inp, coeff, out) of the hdiff variants
  * (stencils/cuda_hip/horizontal_diffusion.py:50-128 and
  * templates/horizontal_diffusion_*.j2); semantics of base.py:276-311.
- * Requires hi, hj >= 2.  Variants: 0 default (= 1), 1 LDS-staged tile
+ * Requires hi, hj >= 2.  Variants: 0 default (1 where positive i and j
+ * strides and a k-plane below 4 GiB let the tile use 32-bit plane offsets,
+ * else 3), 1 LDS-staged tile
  * (launch->by = tile rows 8/16/32, ->bz = k-planes per workgroup 1..16, 0
  * measured default; ->unroll 0 = default (non-temporal coeff loads), else
  * 4 | flags with bit 0 non-temporal out stores, bit 1 non-temporal coeff
@@ -176,8 +178,15 @@ int sb_vadv(int prec, int all_components, void *const *fields,
  * launch->bx = threads per block, ->unroll = vectors per thread,
  * ->variant bit 0 = non-temporal loads, bit 1 = non-temporal stores. */
 int sb_stream_fill(int prec, void *x, size_t n, double value);
-/* x[i] = uniform [0,1) from a counter-based hash of (seed, i). */
+/* x[i] = uniform [0,1) from a counter-based hash of (seed, i); returns after
+ * the fill completed. */
 int sb_stream_fill_random(int prec, void *x, size_t n, uint64_t seed);
+/* Replaces the setup-time random_fill of every field (tools/parallel.cpp:51-96,
+ * stencils/base.py:66-85): fields[f][i] = the value sb_stream_fill_random
+ * gives with seeds[f], for count <= 16 arrays of n elements, in one launch
+ * on the library stream (asynchronous: ordered before later launches). */
+int sb_fill_random_fields(int prec, void *const *fields, int count, size_t n,
+                          const uint64_t *seeds);
 int sb_stream_kernel(int prec, int op, void *a, void *b, void *c, size_t n,
                      double scalar, const sb_launch *l, double *seconds);
 /* max over i of |x[i] - expected| / |expected| (closed-form check of

@@ -75,15 +75,14 @@ class Subdomain:
 
     # -- initialisation ------------------------------------------------------
     def fill_random(self, seed):
-        """U[0,1) on the device (counter hash), u1 = u0 (shared boundary)."""
+        """U[0,1) on the device (counter hash of (seed * 64 + field index,
+        element)), u1 = u0 (the shared boundary): one asynchronous launch."""
         code = native.SB_F64 if self.dtype == np.float64 else native.SB_F32
         n = self.fields["u0"].span // self.dtype.itemsize
-        for index, name in enumerate(FIELDS):
-            if name == "u1":
-                continue
-            native.call("sb_stream_fill_random", code,
-                        ctypes.c_void_p(self.fields[name].base), n, seed * 64 + index)
-        self.fields["u1"].fill_from(self.fields["u0"])
+        index = {name: (0 if name == "u1" else position) for position, name in enumerate(FIELDS)}
+        bases = (ctypes.c_void_p * len(FIELDS))(*[self.fields[name].base for name in FIELDS])
+        seeds = (ctypes.c_uint64 * len(FIELDS))(*[seed * 64 + index[name] for name in FIELDS])
+        native.call("sb_fill_random_fields", code, bases, len(FIELDS), n, seeds)
 
     def upload(self, host_fields):
         """Upload host fields (dict with u, coeff, upos, utens, utensstage, wcon)."""

@@ -115,6 +115,45 @@ __global__ __launch_bounds__(256) void fill_random_kernel(T *x, size_t n, uint64
     }
 }
 
+// Several arrays of n elements in one launch: blockIdx.y is the array.
+constexpr int kMaxFillFields = 16;
+template <typename T>
+struct FillFields {
+    T *p[kMaxFillFields];
+    uint64_t seed[kMaxFillFields];
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_fields_kernel(FillFields<T> f, size_t n) {
+    T *x = f.p[blockIdx.y];
+    const uint64_t seed = f.seed[blockIdx.y];
+    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += size_t(gridDim.x) * blockDim.x) {
+        const uint64_t z = mix64(seed, i);
+        if constexpr (sizeof(T) == 8)
+            x[i] = T(double(z >> 11) * 0x1.0p-53);
+        else
+            x[i] = T(float(uint32_t(z >> 40)) * 0x1.0p-24f);
+    }
+}
+
+int grid_for(size_t n, Context *ctx);
+
+template <typename T>
+int fill_fields(void *const *fields, int count, size_t n, const uint64_t *seeds, Context *ctx) {
+    FillFields<T> f{};
+    for (int q = 0; q < count; ++q) {
+        SB_REQUIRE(fields[q], "NULL array %d", q);
+        f.p[q] = static_cast<T *>(fields[q]);
+        f.seed[q] = seeds[q];
+    }
+    const unsigned gx = unsigned(std::max(1, grid_for(n, ctx) / std::max(1, count)));
+    hipLaunchKernelGGL(fill_fields_kernel<T>, dim3(gx, unsigned(count)), dim3(256), 0, ctx->stream,
+                       f, n);
+    SB_HIP(hipGetLastError());
+    return 0;
+}
+
 // Max relative deviation from `expected`, as the bit pattern of a
 // non-negative double (monotonic under unsigned comparison).
 template <typename T>
@@ -247,6 +286,19 @@ int sb_stream_fill_random(int prec, void *x, size_t n, uint64_t seed) {
     SB_HIP(hipGetLastError());
     SB_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
+}
+
+int sb_fill_random_fields(int prec, void *const *fields, int count, size_t n,
+                          const uint64_t *seeds) {
+    SB_REQUIRE(prec == SB_F64 || prec == SB_F32, "fill precision must be SB_F64 or SB_F32");
+    SB_REQUIRE(count >= 0 && count <= kMaxFillFields, "field count %d not in 0..%d", count,
+               kMaxFillFields);
+    SB_REQUIRE((fields && seeds) || !count, "NULL field or seed array");
+    Context *ctx = context();
+    if (!ctx) return 1;
+    if (!n || !count) return 0;
+    return prec == SB_F64 ? fill_fields<double>(fields, count, n, seeds, ctx)
+                          : fill_fields<float>(fields, count, n, seeds, ctx);
 }
 
 int sb_stream_kernel(int prec, int op, void *a, void *b, void *c, size_t n, double scalar,

@@ -90,6 +90,7 @@ SIGNATURES = {
     "sb_vadv": [_I, _I, ctypes.POINTER(_P), _GP, _LP, _DP],
     "sb_stream_fill": [_I, _P, _SZ, _D],
     "sb_stream_fill_random": [_I, _P, _SZ, ctypes.c_uint64],
+    "sb_fill_random_fields": [_I, ctypes.POINTER(_P), _I, _SZ, ctypes.POINTER(ctypes.c_uint64)],
     "sb_stream_kernel": [_I, _I, _P, _P, _P, _SZ, _D, _LP, _DP],
     "sb_stream_check": [_I, _P, _SZ, _D, _DP],
     "sb_batch_step": [_I, _P, _I, _LP, _LP, _DP],

@@ -358,6 +358,33 @@ def test_stream_kernels(gpu, dtype, op, unroll, nt):
         assert np.array_equal(g, e)
 
 
+def test_fill_random_fields_matches_host_hash(gpu):
+    from stencil_benchmarks_amd.device import DeviceBuffer
+    from stencil_benchmarks_amd.tools.hashfill import uniform
+
+    n = 3001
+    for dtype, code in (("float64", gpu.SB_F64), ("float32", gpu.SB_F32)):
+        bufs = [DeviceBuffer(n * np.dtype(dtype).itemsize) for _ in range(5)]
+        seeds = [7, 7, 1 << 40, 3, 0]  # a repeated seed gives identical arrays
+        ptrs = (ctypes.c_void_p * 5)(*[b.ptr for b in bufs])
+        gpu.call("sb_fill_random_fields", code, ptrs, 5, n, (ctypes.c_uint64 * 5)(*seeds))
+        gpu.call("sb_synchronize")
+        for buf, seed in zip(bufs, seeds):
+            out = np.empty(n, dtype)
+            gpu.call("sb_memcpy_d2h", ctypes.c_void_p(out.ctypes.data), ctypes.c_void_p(buf.ptr),
+                     out.nbytes)
+            assert np.array_equal(out, uniform(seed, np.arange(n), dtype)), seed
+
+
+def test_subdomain_fill_shares_the_boundary(gpu):
+    from stencil_benchmarks_amd import batch
+
+    sub = batch.Subdomain((37, 20, 9), (3, 3, 3), "float32", seed=5)
+    u0, u1, wcon = sub.download("u0"), sub.download("u1"), sub.download("wcon")
+    assert np.array_equal(u0, u1) and not np.array_equal(u0, wcon)
+    sub.free()
+
+
 def test_stream_random_fill_matches_host_hash(gpu):
     from stencil_benchmarks_amd.device import DeviceBuffer
     from stencil_benchmarks_amd.tools.hashfill import uniform
