@@ -55,17 +55,15 @@ class Subdomain:
         self.domain = tuple(int(d) for d in domain)
         self.halo = tuple(int(h) for h in halo)
         self.dtype = np.dtype(dtype)
-        shape = tuple(d + 2 * h for d, h in zip(self.domain, self.halo))
-        self.template = array.alloc_array(shape, self.dtype, layout, alignment, self.halo)
-        self.strides = tuple(s // self.dtype.itemsize for s in self.template.strides)
+        # the geometry of alloc_array (layout, innermost padding), no host array
+        self.shape = tuple(d + 2 * h for d, h in zip(self.domain, self.halo))
+        self.byte_strides, _ = array.compute_strides(self.shape, self.dtype.itemsize,
+                                                     tuple(layout), alignment)
+        self.strides = tuple(s // self.dtype.itemsize for s in self.byte_strides)
         self.geom = geometry(self.domain, self.strides, self.halo)
-        self.fields = {}
-        try:
-            for name in FIELDS:  # from `arena` if given (a DeviceArena), else own allocations
-                self.fields[name] = DeviceField(self.template, self.halo, arena=arena)
-        except BaseException:
-            self.free()
-            raise
+        # all seven fields in one allocation (a block of `arena` if given)
+        self.fields = dict(zip(FIELDS, DeviceField.group(
+            self.shape, self.dtype, self.byte_strides, len(FIELDS), self.halo, arena)))
         self.parity = 0
         self.steps_done = 0
         if host_fields is not None:
@@ -91,16 +89,15 @@ class Subdomain:
             self.fields[name].upload(self._conform(source))
 
     def _conform(self, host):
-        if host.strides == self.template.strides and host.dtype == self.dtype:
+        if host.strides == self.byte_strides and host.dtype == self.dtype:
             return host
         out = self.empty_host()
         out[...] = host
         return out
 
     def empty_host(self):
-        shape = self.template.shape
         buf = np.zeros(self.fields["u0"].span, dtype=np.uint8)
-        return np.ndarray(shape, self.dtype, buffer=buf, strides=self.template.strides)
+        return np.ndarray(self.shape, self.dtype, buffer=buf, strides=self.byte_strides)
 
     def download(self, name):
         host = self.empty_host()

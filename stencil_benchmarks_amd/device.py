@@ -125,6 +125,30 @@ class DeviceField:
         self.base = base + ((-(base + self.origin)) % DEVICE_ALIGNMENT)
         self.index_to_align = index
 
+    @classmethod
+    def group(cls, shape, dtype, strides, count, index_to_align=None, arena=None):
+        """`count` fields of one geometry (`strides` in bytes, non-negative)
+        carved from one allocation (a block of `arena` if given): the
+        geometry is computed once, and each field's slot keeps the same
+        256-byte alignment of `index_to_align`."""
+        dtype = np.dtype(dtype)
+        shape, strides = tuple(shape), tuple(strides)
+        index = tuple(index_to_align) if index_to_align is not None else (0,) * len(shape)
+        span = int(sum((n - 1) * st for n, st in zip(shape, strides)) + dtype.itemsize)
+        origin = int(sum(st * i for st, i in zip(strides, index)))
+        slot = -(-(span + DEVICE_ALIGNMENT) // DEVICE_ALIGNMENT) * DEVICE_ALIGNMENT
+        buffer = arena.alloc(count * slot) if arena is not None else DeviceBuffer(count * slot)
+        fields = []
+        for q in range(count):
+            field = cls.__new__(cls)
+            field.shape, field.dtype, field.strides = shape, dtype, strides
+            field.span, field.origin, field.index_to_align = span, origin, index
+            field.buffer = buffer  # shared; freeing it once frees every field
+            base = buffer.ptr + q * slot
+            field.base = base + ((-(base + origin)) % DEVICE_ALIGNMENT)
+            fields.append(field)
+        return fields
+
     def pointer(self, index=None) -> int:
         """Device address of element `index` (default: the aligned index)."""
         index = self.index_to_align if index is None else tuple(index)
