@@ -64,6 +64,13 @@ class Subdomain:
         # all seven fields in one allocation (a block of `arena` if given)
         self.fields = dict(zip(FIELDS, DeviceField.group(
             self.shape, self.dtype, self.byte_strides, len(FIELDS), self.halo, arena)))
+        nx, ny, nz = self.domain
+        self.points = nx * ny * nz
+        s = self.dtype.itemsize
+        # HBM bytes one step must move: hdiff (inp + halo, coeff, out) + vadv
+        self._step_bytes = (((nx + 4) * (ny + 4) * nz + 2 * nx * ny * nz) * s
+                            + (5 * nx * ny * nz + (nx + 1) * ny * nz) * s)
+        self._packed = {}  # parity -> bytes of the SubdomainStruct
         self.parity = 0
         self.steps_done = 0
         if host_fields is not None:
@@ -122,21 +129,21 @@ class Subdomain:
         s.parity = self.parity
         return s
 
+    def packed(self):
+        """The SubdomainStruct as bytes (cached per parity: the fields and
+        geometry never change)."""
+        blob = self._packed.get(self.parity)
+        if blob is None:
+            blob = self._packed[self.parity] = bytes(self.struct())
+        return blob
+
     def advance(self, steps=1):
         self.parity = (self.parity + steps) & 1
         self.steps_done += steps
 
-    @property
-    def points(self):
-        return int(np.prod(self.domain))
-
     def compulsory_bytes_per_step(self):
         """HBM bytes one step must move: hdiff (inp + halo, coeff, out) + vadv."""
-        nx, ny, nz = self.domain
-        s = self.dtype.itemsize
-        hdiff = ((nx + 4) * (ny + 4) * nz + 2 * nx * ny * nz) * s
-        vadv = (5 * nx * ny * nz + (nx + 1) * ny * nz) * s
-        return hdiff + vadv
+        return self._step_bytes
 
     def free(self):
         for field in self.fields.values():
@@ -144,10 +151,10 @@ class Subdomain:
 
 
 def _array(subs):
-    arr = (SubdomainStruct * max(1, len(subs)))()
-    for i, sub in enumerate(subs):
-        arr[i] = sub.struct()
-    return arr
+    n = max(1, len(subs))
+    blob = b"".join(sub.packed() for sub in subs)
+    return (SubdomainStruct * n).from_buffer_copy(blob.ljust(n * ctypes.sizeof(SubdomainStruct),
+                                                            b"\0"))
 
 
 def precision_of(subs, compute):
