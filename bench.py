@@ -326,7 +326,8 @@ def main():
     parser.add_argument("--steps", type=int, default=20)
     parser.add_argument("--warmup", type=int, default=3)
     parser.add_argument("--quick", action="store_true", help="headline only")
-    parser.add_argument("--only", choices=["hdiff", "vadv", "stream", "batched", "serving"],
+    parser.add_argument("--only", choices=["hdiff", "vadv", "laplacian", "stream", "batched",
+                                           "serving"],
                         help="run a single workload (profiling)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = parser.parse_args()
@@ -442,6 +443,8 @@ def run_single(args):
             (512, 512, 160), np.float64, native.SB_F64, args.steps)
         out["vadv_512x512x160_f32s_f64c_gbs"], out["mean_s_f32s"] = bench_vadv(
             (512, 512, 160), np.float32, native.SB_F32_F64, args.steps)
+    elif args.only == "laplacian":
+        out["laplacian_gbs"], out["mean_s"] = bench_lap(args.steps)
     elif args.only == "stream":
         out["stream_gbs"] = bench_stream(args.steps)
     elif args.only == "batched":

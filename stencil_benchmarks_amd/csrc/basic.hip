@@ -17,6 +17,7 @@
 //   symmetric (inp[x+e] + inp[x-e]) / 2                     (base.py:221)
 //   laplacian r = 0; r += (2 inp - inp[x+e]) - inp[x-e] per enabled axis in
 //             x, y, z order                                 (base.py:243-253)
+#include <algorithm>
 #include <type_traits>
 
 #include "sb_internal.h"
@@ -176,7 +177,7 @@ template <typename T, typename C, int OP, int V, int JB>
 __global__ __launch_bounds__(BLOCK) void march_kernel(const T *__restrict__ inp,
                                                       T *__restrict__ out, BasicArgs a) {
     using VT = Vec<T, V>;
-    const int64_t i = (int64_t(blockIdx.y) * BLOCK + threadIdx.x) * V;
+    const int64_t i = (int64_t(blockIdx.y) * blockDim.x + threadIdx.x) * V;
     if (i >= a.n[0]) return;
     const int64_t chunks = (a.n[1] + JB - 1) / JB;
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
@@ -340,19 +341,40 @@ int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis
                "term along the largest-stride axis");
     const int64_t iblocks = (a.n[0] + int64_t(BLOCK) * V - 1) / (int64_t(BLOCK) * V);
     if (variant == 2 || (variant == 0 && marchable && axis1)) {
-        // launch->bz: rows per thread (8, 16, 32, 64; 0 = 8: about five
-        // rounds of resident workgroups at 512x512x160, a short last round
-        // costs less than with 1.25 rounds at 32 rows)
-        const int JB = (l && l->bz) ? l->bz : 8;
+        // launch->bz: rows per thread (8, 16, 32, 64).  0: the shortest run
+        // with which every workgroup is resident at once (a short last round
+        // of workgroups idles most of the chip), else 64.  Each run re-reads
+        // its two halo rows, which neighbouring runs rarely still hold in L2:
+        // at 512x512x160 FP64, 8-row runs fetch 1.66x the input bytes.
+        // threads per workgroup: one row segment of up to BLOCK x V points,
+        // no wider than the row (whole waves)
+        const int threads =
+            int(std::min<int64_t>(BLOCK, ((a.n[0] + V - 1) / V + 63) / 64 * 64));
+        const int64_t mblocks = (a.n[0] + int64_t(threads) * V - 1) / (int64_t(threads) * V);
+        int JB = (l && l->bz) ? l->bz : 0;
+        if (JB == 0) {
+            int per_cu = 0;
+            const void *probe = reinterpret_cast<const void *>(
+                V == 4 && sizeof(T) == 4 ? &march_kernel<T, C, OP, (sizeof(T) == 4 ? 4 : 2), 8>
+                                         : &march_kernel<T, C, OP, 2, 8>);
+            SB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, probe, threads, 0));
+            const int64_t resident = int64_t(std::max(per_cu, 1)) * ctx->compute_units;
+            JB = 64;
+            for (int rows : {8, 16, 32})
+                if (((a.n[1] + rows - 1) / rows) * a.n[2] * mblocks <= resident) {
+                    JB = rows;
+                    break;
+                }
+        }
         SB_REQUIRE(JB == 8 || JB == 16 || JB == 32 || JB == 64,
                    "rows per thread %d not in {8, 16, 32, 64}", JB);
         const int64_t chunks = (a.n[1] + JB - 1) / JB;
         SB_REQUIRE(chunks * a.n[2] < (int64_t(1) << 31), "domain too large for one launch");
-        const dim3 grid(unsigned(chunks * a.n[2]), unsigned(iblocks));
+        const dim3 grid(unsigned(chunks * a.n[2]), unsigned(mblocks));
         auto launch_j = [&](auto v_tag) {
             constexpr int VV = decltype(v_tag)::value;
             auto go = [&](auto kernel) {
-                hipLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, ctx->stream, inp, out, a);
+                hipLaunchKernelGGL(kernel, grid, dim3(threads), 0, ctx->stream, inp, out, a);
             };
             if (JB == 8) go(march_kernel<T, C, OP, VV, 8>);
             else if (JB == 16) go(march_kernel<T, C, OP, VV, 16>);
