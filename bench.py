@@ -184,12 +184,16 @@ def bench_hdiff_f32(steps):
 
 
 def bench_batched(compute_dtype, storage_dtype, repeats=2):
-    """96 uneven subdomains x 10 steps of hdiff then vadv, one hipGraph."""
-    from stencil_benchmarks_amd import batch
+    """96 uneven subdomains x 10 steps of hdiff then vadv, one hipGraph, with
+    launch shapes from the autotuner (tuned on scratch subdomains of the
+    same shapes before the clock, or read from its cache)."""
+    from stencil_benchmarks_amd import autotune, batch
 
     specs = batch.random_specs(96, 32, 512, 40, 160, seed=0)
+    tuned = autotune.tune_batch(specs, storage_dtype, compute_dtype)
     subs = [batch.Subdomain(s.domain, s.halo, storage_dtype, seed=s.seed) for s in specs]
-    plan = batch.Plan(subs, steps=10, compute=compute_dtype)
+    plan = batch.Plan(subs, steps=10, compute=compute_dtype, hdiff_options=tuned.hdiff_options,
+                      vadv_options=tuned.vadv_options)
     plan.run(use_graph=True)  # capture + warm-up
     native.call("sb_flush_caches")
     times = [plan.run(use_graph=True) for _ in range(repeats)]
@@ -201,7 +205,9 @@ def bench_batched(compute_dtype, storage_dtype, repeats=2):
         s.free()
     return {"seconds": seconds, "gbs": nbytes / seconds / 1e9,
             "gridpoint_steps_per_s": points / seconds, "subdomains": len(subs),
-            "steps": 10, "points_per_step": points // 10}
+            "steps": 10, "points_per_step": points // 10,
+            "autotuned": {"hdiff": tuned.hdiff, "vadv": tuned.vadv,
+                          "from_cache": tuned.from_cache, "candidates": len(tuned.trials)}}
 
 
 def bench_plain_steps(steps=10):

@@ -1,0 +1,39 @@
+"""The autotuner on the device: it picks options from its spaces, caches
+them, and the tuned options give bit-identical results to the defaults."""
+
+import numpy as np
+import pytest
+
+from stencil_benchmarks_amd import autotune, batch
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [batch.SubdomainSpec((70, 37, 50)), batch.SubdomainSpec((130, 20, 97)),
+         batch.SubdomainSpec((33, 66, 40))]
+
+
+@pytest.mark.parametrize("storage", ["float64", "float32"])
+def test_tune_batch_caches_and_keeps_results(gpu, tmp_path, storage):
+    cache = autotune.TuneCache(tmp_path / "tune.json")
+    tuned = autotune.tune_batch(SPECS, storage, "float64", cache=cache)
+    assert not tuned.from_cache and tuned.seconds > 0
+    assert tuned.hdiff in autotune.BATCH_HDIFF_SPACE and tuned.vadv in autotune.BATCH_VADV_SPACE
+    again = autotune.tune_batch(SPECS, storage, "float64", cache=autotune.TuneCache(cache.path))
+    assert again.from_cache and (again.hdiff, again.vadv) == (tuned.hdiff, tuned.vadv)
+
+    results = []
+    for options in ((None, None), (tuned.hdiff_options, tuned.vadv_options)):
+        subs = [batch.Subdomain(s.domain, s.halo, storage, seed=i) for i, s in enumerate(SPECS)]
+        for _ in range(2):
+            batch.step(subs, compute="float64", hdiff_options=options[0], vadv_options=options[1])
+        results.append([(s.state(), s.download("utensstage")) for s in subs])
+        for s in subs:
+            s.free()
+    for (u0, t0), (u1, t1) in zip(*results):
+        assert np.array_equal(u0, u1) and np.array_equal(t0, t1)
+
+
+def test_tune_hdiff_picks_a_tile(gpu, tmp_path):
+    cache = autotune.TuneCache(tmp_path / "tune.json")
+    tuned = autotune.tune_hdiff((96, 80, 24), "float64", cache=cache, repeats=1)
+    assert tuned.hdiff in autotune.PLAIN_HDIFF_SPACE and len(tuned.trials) == 60
