@@ -15,8 +15,10 @@
 // with dtr = 3/20 and bet_m = bet_p = 0.5.
 //
 // Work decomposition.  One lane owns one column; the 64 lanes of the
-// compute wave own 64 consecutive i at fixed j, so every field access of a
-// level is one contiguous 64-element row segment.  The compute wave keeps
+// compute wave own 64 consecutive i at fixed j (variants 1 and 2), or 64
+// consecutive columns in i-fastest order that may run on into the next row
+// (variants 3 and 4, vadv_column.h), so a level's field access is one or
+// two contiguous row segments.  The compute wave keeps
 // stage / wcon of the previous level in registers, loads the next PF levels
 // while it computes the current ones, stores (c', d') of levels 0..nz-2 and
 // runs the backward substitution from that storage in the same kernel.

@@ -51,8 +51,8 @@ __device__ __forceinline__ T stream_load(const T *p) {
 // Variant 3: persistent pipelined column solver.
 //
 // A workgroup (one wave) processes a sequence of 64-column groups g_0, g_1,
-// ... (g_n = blockIdx + n * gridDim, so the groups in flight across the chip
-// are neighbours in memory).  The forward sweep of group n runs in the same
+// ... (g_n = w + n * gridDim for the workgroup's logical index w, so the
+// groups in flight across the chip are neighbours in memory).  The forward sweep of group n runs in the same
 // loop as the backward substitution of group n-1: at step s the backward
 // handles level nz-1-s of group n-1 and the forward level s-1 of group n.
 // Both use ONE LDS buffer of nz-1 (c', d') slots per lane: group n stores
