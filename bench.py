@@ -68,9 +68,12 @@ def device_fields_random(domain, halo, dtype, count, seed, alignment=64):
 
 
 def time_launches(launch, steps, warmup, flush=True):
-    """Warm up, then time `steps` launches; returns per-launch seconds and wall."""
+    """Warm up with `warmup` launches configured like the timed ones (cache
+    flush included, so the flush buffer exists and the clocks have ramped),
+    then time `steps` launches; returns per-launch seconds and wall."""
     for _ in range(warmup):
-        launch(native.launch_options(), None)
+        seconds = ctypes.c_double(0.0)
+        launch(native.launch_options(flush=flush), ctypes.byref(seconds))
     native.call("sb_synchronize")
     times = []
     start = time.perf_counter()
@@ -153,7 +156,7 @@ def bench_lap(steps):
     def launch(opts, seconds):
         native.call("sb_laplacian", native.SB_F64, 1, 1, 0, *ptrs, geom, opts, seconds)
 
-    times, _ = time_launches(launch, steps, 1)
+    times, _ = time_launches(launch, steps, 30)
     return lap_bytes(domain, 8) / times.mean() / 1e9, times.mean()
 
 
@@ -166,7 +169,7 @@ def bench_vadv(domain, dtype, precision, steps, variant=0):
         opts.variant = variant
         native.call("sb_vadv", precision, 0, ptr_array, geom, opts, seconds)
 
-    times, _ = time_launches(launch, steps, 1)
+    times, _ = time_launches(launch, steps, 30)
     itemsize = np.dtype(dtype).itemsize
     return vadv_bytes(domain, itemsize) / times.mean() / 1e9, times.mean()
 
@@ -179,7 +182,7 @@ def bench_hdiff_f32(steps):
     def launch(opts, seconds):
         native.call("sb_hdiff", native.SB_F32_F64, *ptrs, geom, opts, seconds)
 
-    times, _ = time_launches(launch, steps, 1)
+    times, _ = time_launches(launch, steps, 30)
     return hdiff_bytes(domain, 4) / times.mean() / 1e9, times.mean()
 
 
@@ -330,7 +333,10 @@ def main():
     parser = argparse.ArgumentParser()
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=20)
-    parser.add_argument("--warmup", type=int, default=3)
+    # the first ~50 launches of a kernel on fresh fields run about 5 % slower
+    # than the steady state (hdiff 512x512x160: 180.7 us, then 171.5 us;
+    # busy-waiting the GPU beforehand does not remove it)
+    parser.add_argument("--warmup", type=int, default=60)
     parser.add_argument("--quick", action="store_true", help="headline only")
     parser.add_argument("--only", choices=["hdiff", "vadv", "laplacian", "stream", "batched",
                                            "serving"],
@@ -339,7 +345,8 @@ def main():
     args = parser.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the stencil path shards by subdomain, see DESIGN.md)", file=sys.stderr)
+              "(the stencil path would shard by subdomain; one GPU runs one shard)",
+              file=sys.stderr)
         sys.exit(2)
 
     native.init(0)
