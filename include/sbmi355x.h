@@ -160,8 +160,9 @@ int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
  * *tensstage fields are written; ccol, dcol and datacol are never touched
  * (the Thomas coefficients stay on chip, or in a library-owned scratch for
  * variant 2).  Requires hi >= 1 (all components: hj >= 1 too).  Variants:
- * 0 default (4 for nz >= 96 with at least two column groups per resident
- * workgroup, else 3; 2 beyond the LDS limit of nz = 160), 1 one wave per
+ * 0 default (4 for nz >= 96, or nz >= 64 with float32 storage, with at
+ * least two column groups per resident workgroup, else 3; 2 beyond the LDS
+ * limit of nz = 160), 1 one wave per
  * group with the coefficients in LDS, 2 coefficients in a global FP64
  * scratch, 3 single-wave pipeline fusing the forward sweep of one group with
  * the backward of the previous, 4 warp-specialised solver (four waves share

@@ -3,7 +3,8 @@
 One lane per (i, j) column with contiguous i, forward sweep and fused
 backward substitution in one kernel (csrc/vadv.hip).  ``Fused`` picks the
 solver: the warp-specialised one (four waves per 64-column group) for
-nz >= 96, the single-wave pipeline below that, both with the Thomas
+nz >= 96 (nz >= 64 with float32 storage), the single-wave pipeline below
+that, both with the Thomas
 coefficients in LDS (nz <= 160 in FP64 arithmetic), and a global FP64
 scratch beyond.  ``LdsColumn``, ``GlobalScratch``, ``Pipelined`` and
 ``WarpSpecialised`` pin one solver.  Arithmetic follows

@@ -530,13 +530,16 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
                            uint64_t(last_elem + 1) * sizeof(T) < (uint64_t(1) << 32);
     const size_t ws_lds = ws_lds_bytes<C>(g->nz);
     const bool ws_fits = ws_lds <= ctx->lds_per_block;
-    // default: the warp-specialised solver where the single-wave pipeline is
-    // held to one group per CU by LDS (nz >= 96) and every workgroup gets at
-    // least two groups; otherwise the pipelined solver (measured on MI355X:
-    // 512x512x160 FP64 1.10x, float32 storage 1.75x; 256x256x80 0.88x)
+    // default: the warp-specialised solver where every workgroup gets at
+    // least two groups and the columns are tall enough: nz >= 96 for FP64
+    // storage (the single-wave pipeline is then held to one group per CU by
+    // LDS), nz >= 64 for float32 storage (measured on MI355X: 512x512x160
+    // FP64 1.18x, float32 storage 1.9x; 256x256x80 FP64 0.96x, float32
+    // storage 1.20x; 512x512x64 float32 storage 1.24x)
+    const int ws_min_nz = sizeof(T) == 8 ? 96 : 64;
     const bool ws_default = ws_fits && buffer_ok &&
                             uint64_t(last_elem + 1 + g->sk) * sizeof(T) < kWsDropOffset &&
-                            g->nz >= 96 && groups >= 2 * int64_t(ctx->compute_units);
+                            g->nz >= ws_min_nz && groups >= 2 * int64_t(ctx->compute_units);
     if (variant == 0) variant = ws_default ? 4 : lds_fits ? (buffer_ok ? 3 : 1) : 2;
     SB_REQUIRE(variant >= 1 && variant <= 4, "vadv variant %d not available", variant);
     SB_REQUIRE(variant == 2 || variant == 4 || lds_fits,
