@@ -1,4 +1,5 @@
-"""Device arena bookkeeping (first fit, coalescing) - CPU only.
+"""Device arena bookkeeping (first fit, coalescing), grouped field geometry
+and the cached subdomain structs - CPU only.
 
 The arena is exercised over a stand-in buffer (no device memory): blocks
 never overlap, stay inside the arena and aligned, the in-use count matches
@@ -88,3 +89,52 @@ def test_close_frees_the_underlying_allocation():
     arena = _arena(1024)
     arena.close()
     assert arena.buffer.freed
+
+
+def test_grouped_fields_share_geometry_and_do_not_overlap():
+    import numpy as np
+
+    from stencil_benchmarks_amd.device import DeviceField
+    from stencil_benchmarks_amd.tools import array
+
+    arena = _arena(1 << 34, base=(1 << 40) + 64)  # base deliberately not 256-aligned
+    for domain in [(70, 37, 9), (1, 1, 2), (512, 100, 3)]:
+        halo = (3, 3, 3)
+        shape = tuple(d + 2 * h for d, h in zip(domain, halo))
+        template = array.alloc_array(shape, np.float32, (2, 1, 0), 64, halo)
+        single = DeviceField(template, halo, arena=arena)
+        group = DeviceField.group(shape, np.float32, template.strides, 7, halo, arena)
+        spans = sorted((f.base, f.base + f.span) for f in group)
+        for f in group:
+            assert (f.strides, f.span, f.origin) == (single.strides, single.span, single.origin)
+            assert f.pointer() % 256 == 0 and f.pointer() == f.base + f.origin
+        assert all(a[1] <= b[0] for a, b in zip(spans, spans[1:]))
+        lo, hi = group[0].buffer.ptr, group[0].buffer.ptr + group[0].buffer.nbytes
+        assert all(lo <= f.base and f.base + f.span <= hi for f in group)
+        in_use = arena.in_use
+        for f in group:
+            f.free()  # one shared block, released once
+        assert arena.in_use == in_use - group[0].buffer.nbytes
+        single.free()
+
+
+def test_subdomain_struct_cache_matches_fresh_structs():
+    import ctypes
+
+    from stencil_benchmarks_amd import batch
+
+    arena = _arena(1 << 34)
+    subs = [batch.Subdomain(d, (3, 3, 3), "float32", fill=False, arena=arena)
+            for d in [(64, 16, 40), (33, 70, 97), (512, 3, 2)]]
+    subs[1].advance(1)
+    array = batch._array(subs)
+    assert len(array) == 3
+    for i, sub in enumerate(subs):
+        assert bytes(array[i]) == bytes(sub.struct()) and array[i].parity == sub.parity
+    subs[1].advance(1)
+    assert batch._array(subs)[1].parity == 0
+    assert bytes(batch._array([])[0]) == bytes(ctypes.sizeof(batch.SubdomainStruct))
+    nx, ny, nz = subs[0].domain
+    assert subs[0].points == nx * ny * nz
+    assert subs[0].compulsory_bytes_per_step() == (
+        ((nx + 4) * (ny + 4) * nz + 2 * nx * ny * nz) + (5 * nx * ny * nz + (nx + 1) * ny * nz)) * 4
