@@ -166,9 +166,11 @@ int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
  * group with the coefficients in LDS, 2 coefficients in a global FP64
  * scratch, 3 single-wave pipeline fusing the forward sweep of one group with
  * the backward of the previous, 4 warp-specialised solver (four waves share
- * one LDS slot buffer).  launch->bz: bit 0 non-temporal streamed loads,
- * bit 2 (variant 4, SB_F32_F64 only) reciprocal estimate plus Newton steps
- * instead of the correctly rounded division; -1 plain loads. */
+ * one LDS slot buffer; launch->unroll = levels per barrier step, 4 or 8,
+ * 0 = 4, when variant 4 is asked for explicitly).  launch->bz: bit 0
+ * non-temporal streamed loads, bit 2 (variant 4, SB_F32_F64 only) reciprocal
+ * estimate plus Newton steps instead of the correctly rounded division; -1
+ * plain loads. */
 int sb_vadv(int prec, int all_components, void *const *fields,
             const sb_geom *g, const sb_launch *l, double *seconds);
 
@@ -209,7 +211,8 @@ int sb_stream_check(int prec, const void *x, size_t n, double expected,
  * Launch options: hdiff_launch->by / ->bz / ->unroll = tile height / planes
  * per item / non-temporal flags (as sb_hdiff); vadv_launch->variant = 4
  * (default: warp-specialised solver, items of ->by column groups, 0 = 16,
- * dealt to the workgroups by the host in order of decreasing cost) or 3
+ * dealt to the workgroups by the host in order of decreasing cost, ->unroll
+ * = levels per barrier step 4 or 8, 0 = 4) or 3
  * (single-wave pipeline, items from a device counter; ->unroll = prefetch
  * depth 4 or 8, ->bz = 1 non-temporal streamed loads); with variant 4,
  * ->bz bit 2 selects the fast reciprocal (SB_F32_F64 only). */

@@ -97,14 +97,16 @@ class StencilMixin(Benchmark):
             self.launch(pointers, options, ctypes.byref(seconds))
             elapsed = seconds.value
         else:
-            if options.dry_runs:
-                for _ in range(options.dry_runs):
-                    self.launch(pointers, native.launch_options(variant=options.variant), None)
+            # the same launch shape, without the library's own dry runs and flush
+            plain = native.Launch.from_buffer_copy(options)
+            plain.dry_runs, plain.flush = 0, 0
+            for _ in range(options.dry_runs):
+                self.launch(pointers, plain, None)
             if options.flush:
                 native.call("sb_flush_caches")
             native.call("sb_synchronize")
             start = time.perf_counter()
-            self.launch(pointers, native.launch_options(variant=options.variant), None)
+            self.launch(pointers, plain, None)
             native.call("sb_synchronize")
             elapsed = time.perf_counter() - start
         for name in self.written_fields:

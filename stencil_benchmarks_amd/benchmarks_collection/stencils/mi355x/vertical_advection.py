@@ -7,7 +7,8 @@ nz >= 96 (nz >= 64 with float32 storage), the single-wave pipeline below
 that, both with the Thomas
 coefficients in LDS (nz <= 160 in FP64 arithmetic), and a global FP64
 scratch beyond.  ``LdsColumn``, ``GlobalScratch``, ``Pipelined`` and
-``WarpSpecialised`` pin one solver.  Arithmetic follows
+``WarpSpecialised`` pin one solver (the last with ``levels_per_step`` 4 or
+8 per workgroup barrier).  Arithmetic follows
 stencils/base.py:415-473 (bit-exact in FP64; with float32 storage the
 default arithmetic is FP64 throughout).
 """
@@ -17,6 +18,7 @@ import ctypes
 import numpy as np
 
 from .... import native
+from ....benchmark import Parameter
 from ..base import VerticalAdvectionStencil
 from .mixin import StencilMixin
 
@@ -68,3 +70,9 @@ class Pipelined(VerticalAdvectionMixin, VerticalAdvectionStencil):
 
 class WarpSpecialised(VerticalAdvectionMixin, VerticalAdvectionStencil):
     VARIANT = 4
+    levels_per_step = Parameter("levels per workgroup barrier step", 4, choices=[4, 8])
+
+    def launch_options(self):
+        options = super().launch_options()
+        options.unroll = self.levels_per_step
+        return options

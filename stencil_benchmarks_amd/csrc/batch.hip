@@ -178,8 +178,8 @@ void vadv_batch_kernel(const SubDev *__restrict__ subs, const int64_t *__restric
 // schedule); w is the logical worker of this workgroup, numbered so that
 // consecutive workers share an XCD: the schedule hands consecutive items of
 // a subdomain (adjacent groups of a row, which share the cache lines at
-// their edges) to consecutive workers.
-template <typename T, typename C, bool FAST>
+// their edges) to consecutive workers.  MB levels per barrier step.
+template <typename T, typename C, bool FAST, int MB>
 __global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_batch_kernel(
     const SubDev *__restrict__ subs, const int32_t *__restrict__ off,
     const VItem *__restrict__ list, int xcds) {
@@ -198,8 +198,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_batch_kernel(
                          uint32_t(xcds)};
         const WsFields f{d.out, d.pos, d.tens, d.ts, d.wcon, d.span_bytes};
         // u component: wcon neighbour at i+1
-        WsSolver<T, C, 0, FAST>::run(f, d.si, a, WsSeq{r, m, (groups - r + m - 1) / m}, lane0,
-                                     lane0 + ws_rows(d.nz) * 64);
+        WsSolver<T, C, 0, FAST, MB>::run(f, d.si, a, WsSeq{r, m, (groups - r + m - 1) / m}, lane0,
+                                         lane0 + ws_rows(d.nz, MB) * 64);
     }
 }
 
@@ -248,6 +248,12 @@ int hdiff_flags(const sb_launch *l) { return l && l->unroll ? (l->unroll & 3) : 
 int vadv_variant(const sb_launch *l) { return (l && l->variant) ? l->variant : 4; }
 // groups per item of the warp-specialised solver (launch->by, 0 = default)
 int vadv_ws_per_item(const sb_launch *l) { return (l && l->by > 0) ? l->by : kWsGroupsPerItem; }
+// levels per barrier step of the warp-specialised solver (launch->unroll:
+// 4 or 8, 0 = 4; check_lds rejects other values)
+int vadv_ws_block(const sb_launch *l) { return (l && l->unroll == 8) ? 8 : kWsBlock; }
+bool vadv_ws_block_ok(const sb_launch *l) {
+    return !l || l->unroll == 0 || l->unroll == 4 || l->unroll == 8;
+}
 int vadv_pf(const sb_launch *l) { return (l && l->unroll) ? l->unroll : 8; }
 int vadv_nt(const sb_launch *l) { return (l && l->bz) ? (l->bz > 0 ? 1 : 0) : 0; }
 bool vadv_fast(const sb_launch *l) { return l && l->bz > 0 && (l->bz & 4); }
@@ -280,9 +286,11 @@ size_t vadv_lds_bytes(int max_nz) {
     return size_t(max_nz - 1) * 64 * sizeof(CD<C>);
 }
 
-size_t plan_vadv_lds(int prec, int max_nz, int variant) {
+// LDS per workgroup of the batched vadv (`block`: levels per step of variant 4)
+size_t plan_vadv_lds(int prec, int max_nz, int variant, int block) {
     if (variant == 4)
-        return prec == SB_F32 ? ws_lds_bytes<float>(max_nz) : ws_lds_bytes<double>(max_nz);
+        return prec == SB_F32 ? ws_lds_bytes<float>(max_nz, block)
+                              : ws_lds_bytes<double>(max_nz, block);
     return prec == SB_F32 ? vadv_lds_bytes<float>(max_nz) : vadv_lds_bytes<double>(max_nz);
 }
 
@@ -290,7 +298,7 @@ size_t plan_vadv_lds(int prec, int max_nz, int variant) {
 // a CU hold (the warp-specialised kernel: at most 2, its VGPR budget)
 unsigned vadv_workers(const Plan &p, const Context *ctx) {
     const int variant = vadv_variant(&p.vl);
-    const size_t lds = plan_vadv_lds(p.prec, p.max_nz, variant);
+    const size_t lds = plan_vadv_lds(p.prec, p.max_nz, variant, vadv_ws_block(&p.vl));
     int fit = std::max(1, int(ctx->lds_per_block / std::max<size_t>(lds, 1)));
     if (variant == 4) fit = std::min(fit, 2);
     return unsigned(fit * ctx->compute_units);
@@ -306,10 +314,11 @@ unsigned vadv_workers(const Plan &p, const Context *ctx) {
 // (adjacent groups of a row) on adjacent workers, which share an XCD.
 void schedule_ws(Plan &p, int cls, int workers) {
     std::vector<int64_t> item_cost(size_t(p.count));
+    const int block = vadv_ws_block(&p.vl);
     for (int r = 0; r < p.count; ++r) {
         const SubDev &d = p.host_subs[0][r];
         const int64_t groups = vadv_groups(d.nx, d.ny);
-        const int64_t nb = (d.nz + kWsBlock - 1) / kWsBlock;
+        const int64_t nb = (d.nz + block - 1) / block;
         item_cost[size_t(r)] = ((groups + d.vstride - 1) / d.vstride + 1) * (nb + kWsSkew);
     }
     std::vector<int> ranks;
@@ -363,15 +372,16 @@ void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch
         p.ws[c] = Plan::WsClass{};
     }
     if (ws) {
+        const int block = vadv_ws_block(&p.vl);
         for (int r = 0; r < count; ++r) {
             const int nz = int(subs[order[r]].geom.nz);
-            const int c = 2 * plan_vadv_lds(prec, nz, 4) <= ctx->lds_per_block ? 0 : 1;
+            const int c = 2 * plan_vadv_lds(prec, nz, 4, block) <= ctx->lds_per_block ? 0 : 1;
             p.host_vclass[size_t(r)] = int8_t(c);
             p.ws[c].max_nz = std::max(p.ws[c].max_nz, nz);
         }
         for (int c = 0; c < 2; ++c) {
             if (!p.ws[c].max_nz) continue;
-            const size_t lds = plan_vadv_lds(prec, p.ws[c].max_nz, 4);
+            const size_t lds = plan_vadv_lds(prec, p.ws[c].max_nz, 4, block);
             p.ws[c].per_cu = std::max(1, std::min(2, int(ctx->lds_per_block / lds)));
         }
     }
@@ -455,7 +465,8 @@ hipError_t enqueue_step(const Plan &p, int flip, unsigned long long *counters, C
             const unsigned grid = unsigned(c.per_cu * ctx->compute_units);
             // the schedule was built for this grid: one list per workgroup
             if (c.host_off.size() != size_t(grid) + 1) return hipErrorInvalidValue;
-            const size_t wlds = ws_lds_bytes<C>(c.max_nz);
+            const int block = vadv_ws_block(&p.vl);
+            const size_t wlds = ws_lds_bytes<C>(c.max_nz, block);
             auto launch_w = [&](auto kernel) -> hipError_t {
                 (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(wlds));
@@ -466,11 +477,14 @@ hipError_t enqueue_step(const Plan &p, int flip, unsigned long long *counters, C
             hipError_t e = hipSuccess;
             if constexpr (sizeof(T) == 4 && sizeof(C) == 8) {
                 if (vadv_fast(&p.vl))
-                    e = launch_w(vadv_ws_batch_kernel<T, C, true>);
+                    e = block == 8 ? launch_w(vadv_ws_batch_kernel<T, C, true, 8>)
+                                   : launch_w(vadv_ws_batch_kernel<T, C, true, kWsBlock>);
                 else
-                    e = launch_w(vadv_ws_batch_kernel<T, C, false>);
+                    e = block == 8 ? launch_w(vadv_ws_batch_kernel<T, C, false, 8>)
+                                   : launch_w(vadv_ws_batch_kernel<T, C, false, kWsBlock>);
             } else {
-                e = launch_w(vadv_ws_batch_kernel<T, C, false>);
+                e = block == 8 ? launch_w(vadv_ws_batch_kernel<T, C, false, 8>)
+                               : launch_w(vadv_ws_batch_kernel<T, C, false, kWsBlock>);
             }
             if (e != hipSuccess) return e;
         }
@@ -560,7 +574,10 @@ int check_lds(const Plan &p, Context *ctx) {
     SB_REQUIRE(variant == 3 || variant == 4, "batched vadv variant %d not available", variant);
     SB_REQUIRE(!vadv_fast(&p.vl) || (variant == 4 && p.prec == SB_F32_F64),
                "the fast reciprocal needs variant 4 and float32 storage with FP64 arithmetic");
-    const size_t lds = plan_vadv_lds(p.prec, p.max_nz, variant);
+    SB_REQUIRE(variant != 4 || vadv_ws_block_ok(&p.vl),
+               "warp-specialised vadv: levels per step (launch->unroll) must be 4 or 8, got %d",
+               p.vl.unroll);
+    const size_t lds = plan_vadv_lds(p.prec, p.max_nz, variant, vadv_ws_block(&p.vl));
     SB_REQUIRE(lds <= ctx->lds_per_block,
                "tallest column nz = %d needs %zu bytes of LDS, device allows %zu", p.max_nz, lds,
                ctx->lds_per_block);

@@ -337,8 +337,8 @@ void vadv_pipe_kernel(Components comps, const T *__restrict__ wcon, PipeArgs a) 
 
 // Variant 4: warp-specialised persistent solver (vadv_ws.h); four waves per
 // workgroup, group sequence w, w + gridDim, ... for the workgroup's logical
-// index w (consecutive w on one XCD).
-template <typename T, typename C, bool NT, bool FAST>
+// index w (consecutive w on one XCD); MB levels per barrier step.
+template <typename T, typename C, bool NT, bool FAST, int MB>
 __global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_kernel(Components comps,
                                                                const T *__restrict__ wcon,
                                                                PipeArgs a) {
@@ -350,11 +350,11 @@ __global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_kernel(Components comps
     if (first >= a.ngroups) return;
     const WsSeq q{first, int64_t(gridDim.x), (a.ngroups - first + gridDim.x - 1) / gridDim.x};
     C *cbuf = reinterpret_cast<C *>(lds_raw) + (threadIdx.x & 63);
-    C *dbuf = cbuf + ws_rows(a.nz) * 64;
+    C *dbuf = cbuf + ws_rows(a.nz, MB) * 64;
     for (int c = 0; c < a.ncomp; ++c) {
         const Component &cp = comps.c[c];
         const WsFields f{cp.stage, cp.pos, cp.tens, cp.tensstage, wcon, a.span_bytes};
-        WsSolver<T, C, AUX, FAST>::run(f, cp.wshift, a, q, cbuf, dbuf);
+        WsSolver<T, C, AUX, FAST, MB>::run(f, cp.wshift, a, q, cbuf, dbuf);
     }
 }
 
@@ -455,10 +455,10 @@ PersistentGrid persistent_grid(uint64_t groups, uint64_t resident, int xcds) {
     return {unsigned(balanced), unsigned(xcds)};
 }
 
-template <typename T, typename C, bool NT, bool FAST>
+template <typename T, typename C, bool NT, bool FAST, int MB>
 hipError_t launch_ws_k(int per_cu_limit, uint64_t blocks, size_t lds_bytes, const Components &comps,
                        const T *wcon, const PipeArgs &pa, Context *ctx) {
-    const void *fn = reinterpret_cast<const void *>(&vadv_ws_kernel<T, C, NT, FAST>);
+    const void *fn = reinterpret_cast<const void *>(&vadv_ws_kernel<T, C, NT, FAST, MB>);
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 64 * kWsWaves, lds_bytes);
@@ -469,19 +469,29 @@ hipError_t launch_ws_k(int per_cu_limit, uint64_t blocks, size_t lds_bytes, cons
         blocks, uint64_t(ctx->compute_units) * uint64_t(per_cu), ctx->xcds);
     PipeArgs args = pa;
     args.xcds = pg.xcds;
-    hipLaunchKernelGGL((vadv_ws_kernel<T, C, NT, FAST>), dim3(pg.grid), dim3(64 * kWsWaves),
+    hipLaunchKernelGGL((vadv_ws_kernel<T, C, NT, FAST, MB>), dim3(pg.grid), dim3(64 * kWsWaves),
                        lds_bytes, ctx->stream, comps, wcon, args);
     return hipGetLastError();
 }
 
-template <typename T, typename C>
-hipError_t launch_ws(bool nt, bool fast, int per_cu_limit, uint64_t blocks, size_t lds_bytes,
-                     const Components &comps, const T *wcon, const PipeArgs &pa, Context *ctx) {
+template <typename T, typename C, int MB>
+hipError_t launch_ws_m(bool nt, bool fast, int per_cu_limit, uint64_t blocks, size_t lds_bytes,
+                       const Components &comps, const T *wcon, const PipeArgs &pa, Context *ctx) {
     if (fast)
-        return nt ? launch_ws_k<T, C, true, true>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx)
-                  : launch_ws_k<T, C, false, true>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx);
-    return nt ? launch_ws_k<T, C, true, false>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx)
-              : launch_ws_k<T, C, false, false>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx);
+        return nt ? launch_ws_k<T, C, true, true, MB>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx)
+                  : launch_ws_k<T, C, false, true, MB>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx);
+    return nt ? launch_ws_k<T, C, true, false, MB>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx)
+              : launch_ws_k<T, C, false, false, MB>(per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx);
+}
+
+template <typename T, typename C>
+hipError_t launch_ws(int block, bool nt, bool fast, int per_cu_limit, uint64_t blocks,
+                     size_t lds_bytes, const Components &comps, const T *wcon, const PipeArgs &pa,
+                     Context *ctx) {
+    if (block == 8)
+        return launch_ws_m<T, C, 8>(nt, fast, per_cu_limit, blocks, lds_bytes, comps, wcon, pa, ctx);
+    return launch_ws_m<T, C, kWsBlock>(nt, fast, per_cu_limit, blocks, lds_bytes, comps, wcon, pa,
+                                       ctx);
 }
 
 template <int P>
@@ -545,9 +555,17 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
     SB_REQUIRE(variant == 2 || variant == 4 || lds_fits,
                "nz = %lld needs %zu bytes of LDS per workgroup, device allows %zu",
                (long long)g->nz, lds_bytes + kStaticLds, ctx->lds_per_block);
-    SB_REQUIRE(variant != 4 || ws_fits,
+    // variant 4 chosen explicitly: launch->unroll = levels per barrier step
+    // (4 or 8, 0 = 4); the default choice runs 4
+    const bool ws_explicit = l && l->variant == 4;
+    SB_REQUIRE(!ws_explicit || l->unroll == 0 || l->unroll == 4 || l->unroll == 8,
+               "warp-specialised vadv: levels per step (launch->unroll) must be 4 or 8, got %d",
+               l ? l->unroll : 0);
+    const int ws_block = ws_explicit && l->unroll == 8 ? 8 : kWsBlock;
+    const size_t ws_lds_run = ws_lds_bytes<C>(g->nz, ws_block);
+    SB_REQUIRE(variant != 4 || ws_lds_run <= ctx->lds_per_block,
                "nz = %lld needs %zu bytes of LDS per workgroup, device allows %zu",
-               (long long)g->nz, ws_lds, ctx->lds_per_block);
+               (long long)g->nz, ws_lds_run, ctx->lds_per_block);
 
     CD<C> *gscratch = nullptr;
     if (variant == 2) {
@@ -582,8 +600,8 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
                     uint32_t(uint64_t(last_elem + 1) * sizeof(T)), uint32_t(ctx->xcds)};
         const int per_cu_limit = l ? l->bx : 0;
         return timed_launch(l, seconds, [&]() -> hipError_t {
-            return launch_ws<T, C>(flags & 1, fast_div, per_cu_limit, uint64_t(groups), ws_lds,
-                                   comps, wcon, pa, ctx);
+            return launch_ws<T, C>(ws_block, flags & 1, fast_div, per_cu_limit, uint64_t(groups),
+                                   ws_lds_run, comps, wcon, pa, ctx);
         });
     }
     if (variant == 3) {

@@ -24,14 +24,16 @@
 // per-level condition, LDS slots are addressed by one base per block plus
 // constant offsets, and load offsets advance by one saturating add.
 //
-// Schedule.  Levels are processed in blocks of kWsBlock; one workgroup
+// Schedule.  Levels are processed in blocks of M (4 by default, or 8: fewer
+// barriers per level against a pipeline fill and drain of the same three
+// steps, which suits some heights; the launch chooses); one workgroup
 // barrier ends every step.  Iteration n (forward group n, backward group
-// n-1) has nb + 3 steps, nb = ceil(nz / kWsBlock):
+// n-1) has nb + 3 steps, nb = ceil(nz / M):
 //   B  at step t in [0, nb):      block t of group n-1, from the top down;
 //   P1 at step t in [1, nb]:      block t-1 of group n;
 //   P2 at step t in [2, nb+1]:    block t-2 of group n;
 //   C  at step t in [3, nb+2]:    block t-3 of group n.
-// Slots.  The buffer has nzp = nb kWsBlock rows; with pad = nzp - nz, group
+// Slots.  The buffer has nzp = nb M rows; with pad = nzp - nz, group
 // n keeps level k in row pad + k (n even) or nzp - 1 - k (n odd).  Block t
 // of the backward group and block t of the forward group are then the same
 // rows; every wave touches a different block in a step, each block is
@@ -41,7 +43,7 @@
 // the same nb + 3 barriers per iteration.
 //
 // Prefetch.  The loading waves keep a ring of their next levels' loads
-// (statically indexed: the block loop is unrolled ring / kWsBlock times),
+// (statically indexed: the block loop is unrolled ring / M times),
 // refill an entry right after consuming it, and load the first ring of
 // their next group during their idle steps.  Loads and stores are never
 // under a branch, which lets the compiler count the outstanding loads
@@ -56,13 +58,14 @@ namespace sb {
 namespace {
 
 constexpr int kWsWaves = 4;
-constexpr int kWsBlock = 4;  // levels per step
+constexpr int kWsBlock = 4;  // default levels per step (the other choice: 8)
 constexpr int kWsSkew = 3;   // steps from B's read of a block to C's write
 constexpr int kWsRing = 16;  // prefetch ring of the loading waves, levels
 constexpr uint32_t kWsDropOffset = 0xFFFFFF00u;  // past any field (< 4 GiB - 256 B)
 
-__host__ __device__ constexpr int ws_rows(int64_t nz) {
-    return int((nz + kWsBlock - 1) / kWsBlock) * kWsBlock;
+// slot rows for nz levels in blocks of `block` levels
+__host__ __device__ constexpr int ws_rows(int64_t nz, int block = kWsBlock) {
+    return int((nz + block - 1) / block) * block;
 }
 
 // Workgroup barrier ordering LDS only (global loads stay in flight).
@@ -118,9 +121,10 @@ __device__ __forceinline__ Rsrc ws_rsrc(const WsFields &f) {
                 make_rsrc(f.wcon, f.span_bytes)};
 }
 
-template <typename T, typename C, int AUX, bool FAST>
+template <typename T, typename C, int AUX, bool FAST, int MB = kWsBlock>
 struct WsSolver {
-    static constexpr int M = kWsBlock, R = kWsRing, U = kWsRing / kWsBlock;
+    static constexpr int M = MB, R = kWsRing, U = kWsRing / MB;
+    static_assert(MB == 4 || MB == 8, "levels per step: 4 or 8");
 
     struct Col {  // one group's addressing
         GroupOff g;
@@ -581,8 +585,8 @@ struct WsSolver {
 };
 
 template <typename C>
-constexpr size_t ws_lds_bytes(int64_t nz) {
-    return size_t(2 * ws_rows(nz)) * 64 * sizeof(C);
+constexpr size_t ws_lds_bytes(int64_t nz, int block = kWsBlock) {
+    return size_t(2 * ws_rows(nz, block)) * 64 * sizeof(C);
 }
 
 }  // namespace

@@ -220,24 +220,26 @@ def run_plain(requests, dtype="float64", halo=(3, 3, 3)):
     total_bytes = 0.0
     point_steps = 0.0
     arena = DeviceArena(max(footprint_bytes(r.domain, halo, dtype) for r in requests))
-    native.call("sb_synchronize")
-    start = time.perf_counter()
-    for req in requests:
-        sub = batch.Subdomain(req.domain, halo, dtype, seed=req.seed, arena=arena)
-        f = sub.fields
-        for _ in range(req.steps):
-            cur, nxt = f[f"u{sub.parity}"], f[f"u{sub.parity ^ 1}"]
-            native.call("sb_hdiff", prec, ctypes.c_void_p(cur.pointer()),
-                        ctypes.c_void_p(f["coeff"].pointer()), ctypes.c_void_p(nxt.pointer()),
-                        sub.geom, native.launch_options(), None)
-            ptrs = [nxt, f["upos"], f["utens"], f["utensstage"], f["wcon"]]
-            arr = (ctypes.c_void_p * 8)(*([p.pointer() for p in ptrs] + [0, 0, 0]))
-            native.call("sb_vadv", prec, 0, arr, sub.geom, native.launch_options(), None)
-            sub.advance(1)
-            total_bytes += sub.compulsory_bytes_per_step()
-            point_steps += sub.points
+    try:
         native.call("sb_synchronize")
-        sub.free()
-    elapsed = time.perf_counter() - start
-    arena.close()
+        start = time.perf_counter()
+        for req in requests:
+            sub = batch.Subdomain(req.domain, halo, dtype, seed=req.seed, arena=arena)
+            f = sub.fields
+            for _ in range(req.steps):
+                cur, nxt = f[f"u{sub.parity}"], f[f"u{sub.parity ^ 1}"]
+                native.call("sb_hdiff", prec, ctypes.c_void_p(cur.pointer()),
+                            ctypes.c_void_p(f["coeff"].pointer()), ctypes.c_void_p(nxt.pointer()),
+                            sub.geom, native.launch_options(), None)
+                ptrs = [nxt, f["upos"], f["utens"], f["utensstage"], f["wcon"]]
+                arr = (ctypes.c_void_p * 8)(*([p.pointer() for p in ptrs] + [0, 0, 0]))
+                native.call("sb_vadv", prec, 0, arr, sub.geom, native.launch_options(), None)
+                sub.advance(1)
+                total_bytes += sub.compulsory_bytes_per_step()
+                point_steps += sub.points
+            native.call("sb_synchronize")
+            sub.free()
+        elapsed = time.perf_counter() - start
+    finally:
+        arena.close()
     return elapsed, total_bytes, point_steps

@@ -75,3 +75,24 @@ def test_cli_runs_a_gpu_benchmark_and_writes_csv(gpu, tmp_path):
     table = pd.read_csv(path, index_col=0)
     assert len(table) == 6 and set(table["dtype"]) == {"float32", "float64"}
     assert (table["bandwidth"] > 0).all() and "benchmark-name" in table.columns
+
+
+def test_warp_specialised_levels_per_step_are_checked():
+    from stencil_benchmarks_amd.benchmarks_collection.stencils.mi355x import vertical_advection
+
+    with pytest.raises(benchmark.ParameterError):
+        vertical_advection.WarpSpecialised(levels_per_step=5)
+
+
+# dry runs advance the read-write utensstage, so they run without verification
+@pytest.mark.gpu
+@pytest.mark.parametrize("timers,dry_runs,verify", [("gpu", 0, True), ("wall", 0, True),
+                                                    ("wall", 2, False)])
+def test_warp_specialised_plugin_runs_8_levels_per_step(gpu, timers, dry_runs, verify):
+    from stencil_benchmarks_amd.benchmarks_collection.stencils.mi355x import vertical_advection
+
+    bmark = vertical_advection.WarpSpecialised(levels_per_step=8, timers=timers,
+                                              dry_runs=dry_runs, domain=(96, 40, 37),
+                                              verify=verify)
+    assert bmark.launch_options().unroll == 8
+    assert bmark.run()["time"] > 0

@@ -57,6 +57,8 @@ def test_spaces_hold_only_built_combinations():
         flags = (h.get("unroll", 0) & 3) if h.get("unroll") else 2
         assert (h["by"], h["bz"], flags) in built
     assert len(autotune.PLAIN_HDIFF_SPACE) == 3 * 5 * 4
+    for v in autotune.BATCH_VADV_SPACE:  # levels per step 4 / 8, prefetch depth 4 / 8
+        assert v["variant"] in (3, 4) and v.get("unroll", 0) in (0, 4, 8)
 
 
 def test_launch_from_sets_fields():

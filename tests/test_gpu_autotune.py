@@ -37,3 +37,24 @@ def test_tune_hdiff_picks_a_tile(gpu, tmp_path):
     cache = autotune.TuneCache(tmp_path / "tune.json")
     tuned = autotune.tune_hdiff((96, 80, 24), "float64", cache=cache, repeats=1)
     assert tuned.hdiff in autotune.PLAIN_HDIFF_SPACE and len(tuned.trials) == 60
+
+
+@pytest.mark.parametrize("storage", ["float64", "float32"])
+def test_every_batch_candidate_gives_the_default_results(gpu, storage):
+    # the tuner may pick any candidate: each must compute the same bits
+    def run(hdiff, vadv):
+        subs = [batch.Subdomain(s.domain, s.halo, storage, seed=i) for i, s in enumerate(SPECS)]
+        for _ in range(2):
+            batch.step(subs, compute="float64", hdiff_options=autotune.launch_from(hdiff),
+                       vadv_options=autotune.launch_from(vadv))
+        out = [(s.state(), s.download("utensstage")) for s in subs]
+        for s in subs:
+            s.free()
+        return out
+
+    reference = run({}, {})
+    candidates = ([(h, {}) for h in autotune.BATCH_HDIFF_SPACE]
+                  + [({}, v) for v in autotune.BATCH_VADV_SPACE])
+    for hdiff, vadv in candidates:
+        for (u0, t0), (u1, t1) in zip(reference, run(hdiff, vadv)):
+            assert np.array_equal(u0, u1) and np.array_equal(t0, t1), (hdiff, vadv)

@@ -77,15 +77,18 @@ TALL_DOMAINS = [(100, 20, 160), (64, 8, 133), (40, 40, 50), (3, 70, 97)]
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32_f64"])
-@pytest.mark.parametrize("variant", [3, 4])
+# (variant, unroll): single-wave pipeline; warp-specialised solver at 4 and
+# 8 levels per barrier step
+@pytest.mark.parametrize("variant,unroll", [(3, 0), (4, 0), (4, 8)], ids=["pipe", "ws4", "ws8"])
 @pytest.mark.parametrize("domains", [DOMAINS, TALL_DOMAINS], ids=["short", "tall"])
-def test_batch_steps_match_oracle(gpu, prec, variant, domains):
+def test_batch_steps_match_oracle(gpu, prec, variant, unroll, domains):
     from stencil_benchmarks_amd import batch, native
 
     dtype = np.float64 if prec == "f64" else np.float32
     subs, hosts = _make_batch(dtype, domains=domains)
+    options = native.launch_options(variant=variant, unroll=unroll)
     for _ in range(3):
-        batch.step(subs, compute=np.float64, vadv_options=native.launch_options(variant=variant))
+        batch.step(subs, compute=np.float64, vadv_options=options)
         for sub, hf in zip(subs, hosts):
             oracle_step(hf, sub.domain, sub.halo, dtype)
     _check(subs, hosts, dtype)
@@ -123,6 +126,16 @@ def test_batch_alone_equals_mix(gpu):
         sl = inner(mixed.domain, mixed.halo)
         assert np.array_equal(mixed.state()[sl], single.state()[sl])
         assert np.array_equal(mixed.download("utensstage")[sl], single.download("utensstage")[sl])
+
+
+def test_batch_rejects_other_step_sizes(gpu):
+    from stencil_benchmarks_amd import batch, native
+    from stencil_benchmarks_amd.benchmark import ExecutionError
+
+    sub = batch.Subdomain((64, 2, 20), (3, 3, 3), np.float64, seed=1)
+    with pytest.raises(ExecutionError, match="levels per step"):
+        batch.step([sub], vadv_options=native.launch_options(variant=4, unroll=6))
+    sub.free()
 
 
 def test_batch_rejects_tall_columns(gpu):

@@ -247,14 +247,24 @@ WS_DOMAINS = [(128, 700, 19), (70, 530, 9), (64, 1300, 2), (65, 90, 17), (130, 4
 @pytest.mark.parametrize("prec", PRECISIONS)
 @pytest.mark.parametrize("domain", WS_DOMAINS)
 @pytest.mark.parametrize("nt", [1, -1])
-def test_vadv_warp_specialised(gpu, prec, domain, nt):
+@pytest.mark.parametrize("levels", [0, 8])  # levels per barrier step (0: 4)
+def test_vadv_warp_specialised(gpu, prec, domain, nt, levels):
     halo = (1, 0, 0)
     dtype = _dtype(prec)
     fields = make_fields(domain, halo, dtype, count=8, seed=sum(domain))
     got, = _run_vadv(gpu, fields, [3], domain, halo, _prec(gpu, prec), False, 4,
-                     extra=dict(bz=nt))
+                     extra=dict(bz=nt, unroll=levels))
     expected = _expected(lambda *f: oracle.vadv(*f, domain, halo), fields[:5], prec)
     assert np.array_equal(got[inner(domain, halo)], expected)
+
+
+def test_vadv_warp_specialised_rejects_other_step_sizes(gpu):
+    from stencil_benchmarks_amd.benchmark import ExecutionError
+
+    domain, halo = (64, 2, 8), (1, 0, 0)
+    fields = make_fields(domain, halo, np.float64, count=8, seed=1)
+    with pytest.raises(ExecutionError, match="levels per step"):
+        _run_vadv(gpu, fields, [3], domain, halo, gpu.SB_F64, False, 4, extra=dict(unroll=16))
 
 
 # Tolerance of the fast reciprocal (float32 storage, FP64 arithmetic): the
