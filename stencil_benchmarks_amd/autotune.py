@@ -13,7 +13,10 @@ options on the device and keeps the fastest per device and workload:
   the chosen hdiff), each candidate timed over whole steps on scratch
   subdomains of the workload's geometry (user fields are never advanced);
 * plain horizontal diffusion (`tune_hdiff`): tile height x k-planes x cache
-  hints over one domain.
+  hints over one domain;
+* plain vertical advection (`tune_vadv`): the warp-specialised solver at 4 or
+  8 levels per barrier step or the single-wave pipeline at prefetch depth
+  4/8/16, each with plain or non-temporal streamed loads, over one domain.
 
 Every candidate computes the same arithmetic (the tests check that results
 are bit-identical across launch options), so tuning changes speed only.
@@ -55,6 +58,12 @@ BATCH_VADV_SPACE = (
 # non-temporal coeff loads)
 PLAIN_HDIFF_SPACE = tuple({"variant": 1, "by": by, "bz": bz, "unroll": u}
                           for by in (8, 16, 32) for bz in (1, 2, 4, 8, 16) for u in (0, 4, 5, 7))
+# plain vadv: variant 4 (->unroll levels per barrier step) or 3 (->unroll
+# prefetch depth), ->bz 1 non-temporal streamed loads, -1 plain loads; and
+# variant 2 (global scratch), the one solver for columns beyond the LDS
+PLAIN_VADV_SPACE = tuple({"variant": v, "unroll": u, "bz": bz}
+                         for v, depths in ((4, (4, 8)), (3, (4, 8, 16)))
+                         for u in depths for bz in (1, -1)) + ({"variant": 2, "unroll": 8, "bz": 1},)
 
 
 def default_cache_path():
@@ -228,3 +237,57 @@ def tune_hdiff(domain, storage, compute=None, halo=(3, 3, 3), cache=None, repeat
         fields[0].free()
     cache.put(key, {"hdiff": h, "vadv": {}, "seconds": seconds, "trials": len(trials)})
     return TuneResult(key, h, {}, seconds, False, trials)
+
+
+def tune_vadv(domain, storage, compute=None, halo=(3, 3, 3), cache=None, repeats=3,
+              force=False):
+    """Tuned sb_vadv launch options (u component) for one domain (cold-cache
+    launches).  Solvers that the domain does not fit (LDS, 4 GiB buffer
+    offsets) are skipped."""
+    import ctypes
+
+    from .benchmark import ExecutionError
+    from .device import DeviceField, geometry
+    from .tools import array
+
+    storage = np.dtype(storage)
+    compute = np.dtype(compute or storage)
+    key = device_key() + "|" + workload_key("vadv", [domain], storage, compute, halo)
+    cache = cache if cache is not None else TuneCache()
+    hit = None if force else cache.get(key)
+    if hit:
+        return TuneResult(key, {}, hit["vadv"], hit["seconds"], True)
+    shape = tuple(d + 2 * h for d, h in zip(domain, halo))
+    template = array.alloc_array(shape, storage, (2, 1, 0), 64, halo)
+    strides = [s // storage.itemsize for s in template.strides]
+    # ustage, upos, utens, utensstage, wcon (the reference's ccol, dcol and
+    # datacol are never touched)
+    fields = DeviceField.group(shape, storage, template.strides, 5, halo)
+    seeds = (ctypes.c_uint64 * 5)(1, 2, 3, 4, 5)
+    bases = (ctypes.c_void_p * 5)(*[f.base for f in fields])
+    native.call("sb_fill_random_fields", native.SB_F64 if storage == np.float64 else native.SB_F32,
+                bases, 5, fields[0].span // storage.itemsize, seeds)
+    code = native.precision_code(storage, compute)
+    geom = geometry(domain, strides, halo)
+    pointers = (ctypes.c_void_p * 8)(*([f.pointer() for f in fields] + [0, 0, 0]))
+    try:
+        def measure(_unused, options):
+            launch = launch_from(options)
+            launch.flush = 1
+            seconds = ctypes.c_double()
+            runs = []
+            try:
+                for _ in range(repeats):
+                    native.call("sb_vadv", code, 0, pointers, geom, launch, ctypes.byref(seconds))
+                    runs.append(seconds.value)
+            except ExecutionError:  # this solver does not take the domain
+                return float("inf")
+            return min(runs)
+
+        _, v, seconds, trials = descend(({},), PLAIN_VADV_SPACE, measure, PLAIN_VADV_SPACE[0])
+    finally:
+        fields[0].free()
+    if seconds == float("inf"):
+        raise ExecutionError(f"no vadv solver takes the domain {tuple(domain)}")
+    cache.put(key, {"hdiff": {}, "vadv": v, "seconds": seconds, "trials": len(trials)})
+    return TuneResult(key, {}, v, seconds, False, trials)

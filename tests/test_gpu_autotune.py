@@ -58,3 +58,42 @@ def test_every_batch_candidate_gives_the_default_results(gpu, storage):
     for hdiff, vadv in candidates:
         for (u0, t0), (u1, t1) in zip(reference, run(hdiff, vadv)):
             assert np.array_equal(u0, u1) and np.array_equal(t0, t1), (hdiff, vadv)
+
+
+def _vadv_with(native, fields, domain, halo, options):
+    import ctypes
+
+    from stencil_benchmarks_amd.device import DeviceField, geometry
+
+    devs = [DeviceField(f, halo) for f in fields]
+    for d, f in zip(devs, fields):
+        d.upload(f)
+    geom = geometry(domain, [st // fields[0].itemsize for st in fields[0].strides], halo)
+    pointers = (ctypes.c_void_p * 8)(*([d.pointer() for d in devs] + [0, 0, 0]))
+    native.call("sb_vadv", native.SB_F64, 0, pointers, geom, autotune.launch_from(options), None)
+    native.call("sb_synchronize")
+    out = np.ndarray(fields[3].shape, fields[3].dtype, buffer=np.zeros(devs[3].span, np.uint8),
+                     strides=fields[3].strides)
+    devs[3].download(out)
+    for d in devs:
+        d.free()
+    return out
+
+
+@pytest.mark.parametrize("domain,solver", [((96, 80, 24), None), ((64, 4, 200), 2)])
+def test_tune_vadv_picks_a_solver_that_keeps_results(gpu, tmp_path, domain, solver):
+    from conftest import inner, make_fields
+    from oracle import oracle
+
+    cache = autotune.TuneCache(tmp_path / "tune.json")
+    tuned = autotune.tune_vadv(domain, "float64", cache=cache, repeats=1)
+    assert tuned.vadv in autotune.PLAIN_VADV_SPACE
+    assert len(tuned.trials) == len(autotune.PLAIN_VADV_SPACE)
+    if solver is not None:  # columns beyond the LDS: only the global scratch takes them
+        assert tuned.vadv["variant"] == solver
+    again = autotune.tune_vadv(domain, "float64", cache=autotune.TuneCache(cache.path))
+    assert again.from_cache and again.vadv == tuned.vadv
+    halo = (3, 3, 3)
+    fields = make_fields(domain, halo, np.float64, count=5, seed=7)
+    got = _vadv_with(gpu, fields, domain, halo, tuned.vadv)
+    assert np.array_equal(got[inner(domain, halo)], oracle.vadv(*fields, domain, halo))
