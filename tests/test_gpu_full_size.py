@@ -4,12 +4,12 @@ The other GPU tests sweep small and ragged shapes; these run the shapes the
 measurements are quoted on: horizontal diffusion and the x/y Laplacian at
 512x512x160 (halo 3, 64-byte alignment, layout (2, 1, 0)), vertical
 advection at 512x512x160 and 256x256x80, STREAM at 2**29 elements and the
-96-subdomain batch.  The stencils are compared with the C oracle bit for
-bit (FP64, and float32 storage with FP64 arithmetic against the FP64
-oracle on the float32 inputs, rounded once); STREAM with the reference's
-closed-form recurrence; the batch (too large for the oracle in seconds)
-with the plain kernels, which the oracle pins, applied subdomain by
-subdomain.
+96-subdomain batch, and the served request mix.  The stencils are compared
+with the C oracle bit for bit (FP64, and float32 storage with FP64
+arithmetic against the FP64 oracle on the float32 inputs, rounded once);
+STREAM with the reference's closed-form recurrence; the batch and the
+served requests (too large for the oracle in seconds) with the plain
+kernels, which the oracle pins, applied subdomain by subdomain.
 """
 
 import ctypes
@@ -164,3 +164,27 @@ def test_batch_of_96_subdomains_equals_plain_kernels(gpu, storage):
     finally:
         for sub in batched:
             sub.free()
+
+
+def test_served_request_mix_equals_plain_kernels(gpu):
+    # the serving workload's request mix (nx = ny from 16 to 1024, nz 40 / 80 /
+    # 137, 1-20 steps; float32 storage, FP64 arithmetic), all admitted at once
+    from stencil_benchmarks_amd import batch, serve
+
+    requests = serve.poisson_requests(rate=0.0, count=40, seed=1)
+    server = serve.Server(dtype="float32", compute="float64", keep_results=True)
+    stats = server.run(requests, realtime=False)
+    assert stats.requests == 40 and stats.max_active > 1
+    code = gpu.precision_code(np.float32, np.float64)
+    for req in serve.poisson_requests(rate=0.0, count=40, seed=1):
+        plain = batch.Subdomain(req.domain, (3, 3, 3), "float32", seed=req.seed)
+        try:
+            for _ in range(req.steps):
+                _plain_step(gpu, plain, code)
+            gpu.call("sb_synchronize")
+            sl = inner(req.domain, (3, 3, 3))
+            u, ts = server.results[req.rid]
+            assert np.array_equal(u, plain.state()[sl]), req
+            assert np.array_equal(ts, plain.download("utensstage")[sl]), req
+        finally:
+            plain.free()
