@@ -38,6 +38,7 @@
 // hipGraph and replayed).  Every workgroup exits once the counter passes
 // the item count.
 #include <algorithm>
+#include <cstring>
 #include <memory>
 #include <numeric>
 #include <vector>
@@ -232,6 +233,13 @@ struct Plan {
     // host copies
     std::vector<SubDev> host_subs[2];
     std::vector<int64_t> host_hprefix, host_vprefix;
+    // pinned staging of the device tables, two buffers used in turn: each
+    // upload is one asynchronous copy, and a buffer is refilled only after
+    // the event recorded behind its previous copy
+    void *stage[2] = {nullptr, nullptr};
+    size_t stage_bytes[2] = {0, 0};
+    hipEvent_t stage_done[2] = {nullptr, nullptr};
+    int stage_next = 0;
 };
 
 int hdiff_ty(const sb_launch *l) { return (l && l->by) ? l->by : 16; }
@@ -529,42 +537,62 @@ int upload_tables(Plan &p, Context *ctx) {
                     sizeof(VItem) * c.host_list.size();
     const size_t need = sched + ws_bytes + 256;
     if (p.dev_bytes < need) {
+        // steps enqueued earlier may still read the old tables
+        SB_HIP(hipStreamSynchronize(ctx->stream));
         if (p.dev) SB_HIP(hipFree(p.dev));
         p.dev = nullptr;
         SB_HIP(hipMalloc(&p.dev, need));
         p.dev_bytes = need;
     }
+    // staging buffer of this upload (the device tables' image, same offsets)
+    const int b = p.stage_next;
+    p.stage_next ^= 1;
+    if (p.stage_done[b]) {
+        SB_HIP(hipEventSynchronize(p.stage_done[b]));  // its previous copy has run
+    } else {
+        SB_HIP(hipEventCreateWithFlags(&p.stage_done[b], hipEventDisableTiming));
+    }
+    if (p.stage_bytes[b] < need) {
+        if (p.stage[b]) SB_HIP(hipHostFree(p.stage[b]));
+        p.stage[b] = nullptr;
+        p.stage_bytes[b] = 0;
+        SB_HIP(hipHostMalloc(&p.stage[b], need, hipHostMallocDefault));
+        p.stage_bytes[b] = need;
+    }
+    char *host = static_cast<char *>(p.stage[b]);
     char *base = static_cast<char *>(p.dev);
     p.subs[0] = reinterpret_cast<SubDev *>(base);
     p.subs[1] = reinterpret_cast<SubDev *>(base + subs_bytes);
     p.hprefix = reinterpret_cast<int64_t *>(base + 2 * subs_bytes);
     p.vprefix = reinterpret_cast<int64_t *>(base + 2 * subs_bytes + pre_bytes);
     p.counters = reinterpret_cast<unsigned long long *>(base + 2 * subs_bytes + 2 * pre_bytes);
+    if (p.count) {
+        std::memcpy(host, p.host_subs[0].data(), sizeof(SubDev) * p.count);
+        std::memcpy(host + subs_bytes, p.host_subs[1].data(), sizeof(SubDev) * p.count);
+    }
+    std::memcpy(host + 2 * subs_bytes, p.host_hprefix.data(), pre_bytes);
+    std::memcpy(host + 2 * subs_bytes + pre_bytes, p.host_vprefix.data(), pre_bytes);
+    // the counters are cleared by a kernel before every step sequence
+    std::memset(host + 2 * subs_bytes + 2 * pre_bytes, 0, ctr_bytes);
     size_t at = sched;
     for (Plan::WsClass &c : p.ws) {
         const size_t off_bytes = sizeof(int32_t) * c.host_off.size();
         const size_t list_bytes = sizeof(VItem) * c.host_list.size();
         c.off = reinterpret_cast<int32_t *>(base + at);
+        if (off_bytes) std::memcpy(host + at, c.host_off.data(), off_bytes);
         at += (off_bytes + 7) & ~size_t(7);
         c.list = reinterpret_cast<VItem *>(base + at);
+        if (list_bytes) std::memcpy(host + at, c.host_list.data(), list_bytes);
         at += list_bytes;
-        if (off_bytes)
-            SB_HIP(hipMemcpyAsync(c.off, c.host_off.data(), off_bytes, hipMemcpyHostToDevice,
-                                  ctx->stream));
-        if (list_bytes)
-            SB_HIP(hipMemcpyAsync(c.list, c.host_list.data(), list_bytes, hipMemcpyHostToDevice,
-                                  ctx->stream));
     }
-    if (p.count) {
-        SB_HIP(hipMemcpyAsync(p.subs[0], p.host_subs[0].data(), sizeof(SubDev) * p.count,
-                              hipMemcpyHostToDevice, ctx->stream));
-        SB_HIP(hipMemcpyAsync(p.subs[1], p.host_subs[1].data(), sizeof(SubDev) * p.count,
-                              hipMemcpyHostToDevice, ctx->stream));
-    }
-    SB_HIP(hipMemcpyAsync(p.hprefix, p.host_hprefix.data(), pre_bytes, hipMemcpyHostToDevice,
-                          ctx->stream));
-    SB_HIP(hipMemcpyAsync(p.vprefix, p.host_vprefix.data(), pre_bytes, hipMemcpyHostToDevice,
-                          ctx->stream));
+    // one copy, ordered on the stream before the steps that read the tables
+    // (and after the previous steps, which read the same device tables)
+    SB_HIP(hipMemcpyAsync(p.dev, host, at, hipMemcpyHostToDevice, ctx->stream));
+    SB_HIP(hipEventRecord(p.stage_done[b], ctx->stream));
+    return 0;
+}
+
+int sync_stream(Context *ctx) {
     SB_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
 }
@@ -621,6 +649,13 @@ void destroy_plan(Plan *p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->dev) (void)hipFree(p->dev);
+    for (int b = 0; b < 2; ++b) {
+        if (p->stage_done[b]) {
+            (void)hipEventSynchronize(p->stage_done[b]);
+            (void)hipEventDestroy(p->stage_done[b]);
+        }
+        if (p->stage[b]) (void)hipHostFree(p->stage[b]);
+    }
     delete p;
 }
 
@@ -647,7 +682,8 @@ int sb_batch_plan_create(int prec, const sb_subdomain *subs, int count, int step
     if (vadv_launch) p->vl = *vadv_launch;
     build_tables(prec, subs, count, hdiff_launch, *p, ctx);
     if (check_lds(*p, ctx)) return 2;
-    if (upload_tables(*p, ctx)) {
+    // a plan may be captured into a graph later: start it from an idle stream
+    if (upload_tables(*p, ctx) || sync_stream(ctx)) {
         destroy_plan(p.release());
         return 1;
     }
