@@ -268,7 +268,10 @@ def bench_serving(count=160, load=0.8):
         return serve.Server(dtype="float32", compute="float64",
                             hdiff_options=tuned.hdiff_options, vadv_options=tuned.vadv_options)
 
-    server().run(serve.poisson_requests(rate=0.0, count=8, seed=99), realtime=False)  # warm-up
+    # warm-up: the same closed loop once untimed (the first launches on
+    # fresh fields and a cold device run slower; with the tuning cached the
+    # capacity run would otherwise be the first work of the leg)
+    server().run(serve.poisson_requests(rate=0.0, count=count, seed=1), realtime=False)
     closed = server().run(mix, realtime=False)
     capacity = closed.requests / closed.wall_s
     rate = load * capacity

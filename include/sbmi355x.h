@@ -107,6 +107,15 @@ int sb_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int sb_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int sb_memset(void *dst, int value, size_t bytes);
 int sb_synchronize(void);
+/* Markers on the library stream (the reference times with events inside the
+ * generated launcher, cuda_hip/templates/base.j2:146-183, and has no public
+ * event API): sb_event_record enqueues the event behind all work submitted
+ * so far; sb_event_synchronize waits until that work has completed, without
+ * waiting for work submitted after the record. */
+int sb_event_create(void **event);
+int sb_event_record(void *event);
+int sb_event_synchronize(void *event);
+int sb_event_destroy(void *event);
 /* Stream a scratch buffer larger than L2 + Infinity Cache through the
  * device; the cold-cache counterpart of cycling `data_sets`
  * (stencils/base.py:47-51, :152). */

@@ -251,6 +251,35 @@ int sb_synchronize(void) {
     return 0;
 }
 
+int sb_event_create(void **event) {
+    SB_REQUIRE(event, "NULL event pointer");
+    Context *ctx = context();
+    if (!ctx) return 1;
+    hipEvent_t e = nullptr;
+    SB_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    *event = e;
+    return 0;
+}
+
+int sb_event_record(void *event) {
+    SB_REQUIRE(event, "NULL event");
+    Context *ctx = context();
+    if (!ctx) return 1;
+    SB_HIP(hipEventRecord(static_cast<hipEvent_t>(event), ctx->stream));
+    return 0;
+}
+
+int sb_event_synchronize(void *event) {
+    SB_REQUIRE(event, "NULL event");
+    SB_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+    return 0;
+}
+
+int sb_event_destroy(void *event) {
+    if (event) SB_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return 0;
+}
+
 int sb_flush_caches(void) {
     Context *ctx = context();
     if (!ctx) return 1;

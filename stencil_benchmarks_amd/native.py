@@ -81,6 +81,10 @@ SIGNATURES = {
     "sb_memcpy_d2d": [_P, _P, _SZ],
     "sb_memset": [_P, _I, _SZ],
     "sb_synchronize": [],
+    "sb_event_create": [ctypes.POINTER(_P)],
+    "sb_event_record": [_P],
+    "sb_event_synchronize": [_P],
+    "sb_event_destroy": [_P],
     "sb_flush_caches": [],
     "sb_copy": [_I, _P, _P, _GP, _LP, _DP],
     "sb_onesided_average": [_I, _I, _P, _P, _GP, _LP, _DP],
@@ -178,6 +182,27 @@ def mem_info():
     free, total = ctypes.c_size_t(0), ctypes.c_size_t(0)
     call("sb_mem_info", ctypes.byref(free), ctypes.byref(total))
     return free.value, total.value
+
+
+class Event:
+    """A marker on the library stream: record() behind the work submitted so
+    far, wait() until that work has completed."""
+
+    def __init__(self):
+        handle = ctypes.c_void_p()
+        call("sb_event_create", ctypes.byref(handle))
+        self.handle = handle
+
+    def record(self):
+        call("sb_event_record", self.handle)
+
+    def wait(self):
+        call("sb_event_synchronize", self.handle)
+
+    def close(self):
+        if self.handle:
+            call("sb_event_destroy", self.handle)
+            self.handle = None
 
 
 def launch_options(variant=0, block=(0, 0, 0), unroll=0, dry_runs=0, flush=False) -> Launch:

@@ -15,6 +15,11 @@ its U[0,1) fields.  The server
   batched launch pair (``sb_batch_step``): requests that arrive while
   others are on the GPU join at the next tick and share the persistent
   kernels' work queues with them (continuous batching);
+* keeps one tick in flight: tick t+1 is built and enqueued before the host
+  waits for tick t (a marker event behind each tick), so the per-tick host
+  work overlaps the device's; a request is complete, and retired, once the
+  tick that holds its last step has passed (arena blocks it frees are only
+  reused by work enqueued later on the same stream);
 * records per request the latency from arrival to completion.
 
 Timestep semantics are those of :mod:`stencil_benchmarks_amd.batch`; a
@@ -162,46 +167,57 @@ class Server:
         bytes_moved = 0.0
         point_steps = 0.0
         max_active = 0
+        # one tick in flight while the next is prepared and enqueued: the
+        # host's per-tick work overlaps the device's; `inflight` holds the
+        # marker behind the last enqueued tick and the requests whose last
+        # step it holds (their results are complete once it has passed)
+        markers = [native.Event(), native.Event()]
+        inflight = None
         start = time.perf_counter()
-        while pending or active:
-            now = time.perf_counter() - start
-            while pending and (not realtime or pending[0].arrival <= now):
-                req = pending[0]
-                try:
-                    self._admit(req, now)
-                except MemoryError:
-                    if active:
-                        break  # wait for memory (head of line)
-                    raise
-                pending.popleft()
-                if not realtime:
-                    req.arrival = 0.0
-                active.append(req)
-            if not active:
-                # idle until the next arrival
-                wait = pending[0].arrival - (time.perf_counter() - start)
-                if wait > 0:
-                    time.sleep(min(wait, 0.001))
-                continue
-            max_active = max(max_active, len(active))
-            subs = [r.sub for r in active]
-            batch.step(subs, compute=self.compute, hdiff_options=self.hdiff_options,
-                       vadv_options=self.vadv_options)
-            native.call("sb_synchronize")
-            ticks += 1
-            for r in active:
-                bytes_moved += r.sub.compulsory_bytes_per_step()
-                point_steps += r.sub.points
-                r.remaining -= 1
-            now = time.perf_counter() - start
-            still = []
-            for r in active:
-                if r.remaining == 0:
-                    self._retire(r, now)
-                    done.append(r)
-                else:
-                    still.append(r)
-            active = still
+        try:
+            while pending or active or inflight:
+                now = time.perf_counter() - start
+                while pending and (not realtime or pending[0].arrival <= now):
+                    req = pending[0]
+                    try:
+                        self._admit(req, now)
+                    except MemoryError:
+                        if active or inflight:
+                            break  # wait for memory (head of line)
+                        raise
+                    pending.popleft()
+                    if not realtime:
+                        req.arrival = 0.0
+                    active.append(req)
+                tick = None
+                if active:
+                    max_active = max(max_active, len(active))
+                    batch.step([r.sub for r in active], compute=self.compute,
+                               hdiff_options=self.hdiff_options, vadv_options=self.vadv_options)
+                    marker = markers[ticks & 1]
+                    marker.record()
+                    ticks += 1
+                    for r in active:
+                        bytes_moved += r.sub.compulsory_bytes_per_step()
+                        point_steps += r.sub.points
+                        r.remaining -= 1
+                    tick = (marker, [r for r in active if r.remaining == 0])
+                    active = [r for r in active if r.remaining > 0]
+                if inflight is not None:
+                    inflight[0].wait()
+                    now = time.perf_counter() - start
+                    for r in inflight[1]:
+                        self._retire(r, now)
+                        done.append(r)
+                inflight = tick
+                if inflight is None and not active and pending:
+                    # idle until the next arrival
+                    wait = pending[0].arrival - (time.perf_counter() - start)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.001))
+        finally:
+            for marker in markers:
+                marker.close()
         wall = time.perf_counter() - start
         return ServeStats(len(done), wall, ticks, [r.latency for r in done], bytes_moved,
                           point_steps, max_active)

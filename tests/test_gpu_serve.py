@@ -32,6 +32,46 @@ def test_serve_results_alone_equal_mix(gpu, dtype):
         assert np.array_equal(u_mix, u_one) and np.array_equal(ts_mix, ts_one), req
 
 
+def test_serve_realtime_arrivals_keep_results(gpu):
+    # arrivals spread over time (idle waits, joins at later ticks, a tick in
+    # flight while the next is enqueued): same results as the closed loop
+    from stencil_benchmarks_amd import serve
+
+    def stream():
+        return serve.poisson_requests(rate=3000.0, count=12, seed=5, nmin=8, nmax=96,
+                                      nz_choices=(2, 9, 40), smin=1, smax=6)
+
+    timed = serve.Server(dtype="float32", keep_results=True)
+    stats = timed.run(stream(), realtime=True)
+    assert stats.requests == 12 and stats.ticks >= 6
+    assert all(lat > 0 for lat in stats.latencies_s)
+    closed = serve.Server(dtype="float32", keep_results=True)
+    closed.run(stream(), realtime=False)
+    for rid, (u, ts) in closed.results.items():
+        assert np.array_equal(u, timed.results[rid][0]) and np.array_equal(ts, timed.results[rid][1])
+
+
+def test_events_order_host_waits(gpu):
+    import ctypes
+
+    from stencil_benchmarks_amd.device import DeviceBuffer
+
+    n = 1 << 20
+    buf = DeviceBuffer(8 * n)
+    marker = gpu.Event()
+    try:
+        gpu.call("sb_stream_fill", gpu.SB_F64, ctypes.c_void_p(buf.ptr), n, 2.5)
+        marker.record()
+        marker.wait()
+        worst = ctypes.c_double(1.0)
+        gpu.call("sb_stream_check", gpu.SB_F64, ctypes.c_void_p(buf.ptr), n, 2.5,
+                 ctypes.byref(worst))
+        assert worst.value == 0.0
+    finally:
+        marker.close()
+        buf.free()
+
+
 def test_serve_matches_oracle(gpu):
     from stencil_benchmarks_amd import batch, serve
     from test_gpu_batch import oracle_step
