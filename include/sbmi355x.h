@@ -178,8 +178,8 @@ int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
  * one LDS slot buffer; launch->unroll = levels per barrier step, 4 or 8,
  * 0 = 4, when variant 4 is asked for explicitly).  launch->bz: bit 0
  * non-temporal streamed loads, bit 2 (variant 4, SB_F32_F64 only) reciprocal
- * estimate plus Newton steps instead of the correctly rounded division; -1
- * plain loads. */
+ * estimate plus Newton steps instead of the correctly rounded division; 0
+ * and -1 plain loads. */
 int sb_vadv(int prec, int all_components, void *const *fields,
             const sb_geom *g, const sb_launch *l, double *seconds);
 

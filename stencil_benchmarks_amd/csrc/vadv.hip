@@ -576,19 +576,18 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
             return 1;
         gscratch = static_cast<CD<C> *>(g_scratch[ctx->device & 63]);
     }
-    // defaults measured on MI355X: deep prefetch for tall columns, and
-    // non-temporal streamed loads when the fields are not much larger than
-    // the 256 MB Infinity Cache (they keep pos resident for the backward)
-    const double field_bytes = double(g->nx) * double(g->ny) * double(g->nz) * sizeof(T);
+    // defaults measured on MI355X: deep prefetch for tall columns, plain
+    // loads (non-temporal loads of the streamed fields measured 0-5 % slower
+    // on 128²..512² x 32..80 in both storage types, 2 % faster only at
+    // 256²x40 FP64)
     const int depth = (l && l->unroll) ? l->unroll : (g->nz >= 128 ? 8 : 4);
     SB_REQUIRE(depth == 4 || depth == 8 || depth == 16, "vadv prefetch depth %d not in {4, 8, 16}",
                depth);
     // launch->bz: bit 0 non-temporal streamed loads, bit 1 helper wave (variant 1),
     // bit 2 reciprocal estimate + Newton steps instead of division (variant 4,
-    // float32 storage with FP64 arithmetic only); 0 selects the default, -1
-    // forces plain loads
+    // float32 storage with FP64 arithmetic only); 0 (the default) and -1 are
+    // plain loads
     int flags = l ? l->bz : 0;
-    if (flags == 0) flags = (variant != 4 && 6.0 * field_bytes < 512e6) ? 1 : 0;
     if (flags < 0) flags = 0;
     const bool fast_div = (flags & 4) != 0;
     SB_REQUIRE(!fast_div || (variant == 4 && P == SB_F32_F64),
