@@ -387,10 +387,17 @@ def main():
     achieved = head["bytes"] / mean_s / 1e9
     log(f"hdiff: {achieved:.1f} GB/s, mean launch {mean_s * 1e6:.1f} us")
 
+    # the reference's own report for the same launches: bandwidth by its
+    # data_size formula (2N + (nx+4)(ny+4)(nz+4)) * itemsize, and grid points/s
+    nx, ny, nz = 512, 512, 160
+    points = nx * ny * nz
     extras = {"hdiff_512x512x160_f64_pcie_inclusive": {
         "gbs": head["bytes"] / head["round_trip"] / 1e9, "seconds": head["round_trip"],
         "transfer_bytes": head["transfer_bytes"],
-        "note": "host inp + coeff uploaded, one launch, out downloaded (pageable buffers)"}}
+        "note": "host inp + coeff uploaded, one launch, out downloaded (pageable buffers)"},
+        "hdiff_512x512x160_f64_reference_formula_gbs":
+            (2 * points + (nx + 4) * (ny + 4) * (nz + 4)) * 8 / mean_s / 1e9,
+        "hdiff_512x512x160_f64_gridpoints_per_s": points / mean_s}
     if not args.quick:
         log("STREAM 3 x 2^29 float64")
         extras["stream_gbs"] = bench_stream(args.steps // 4 + 3)
@@ -398,8 +405,11 @@ def main():
         log("laplacian 512x512x160 float64")
         extras["laplacian_gbs"], _ = bench_lap(args.steps)
         log("vadv")
-        extras["vadv_512x512x160_f64_gbs"], _ = bench_vadv((512, 512, 160), np.float64,
-                                                           native.SB_F64, args.steps)
+        extras["vadv_512x512x160_f64_gbs"], vadv_s = bench_vadv((512, 512, 160), np.float64,
+                                                                native.SB_F64, args.steps)
+        # the reference's vadv data_size counts ccol/dcol traffic: 10 N * itemsize
+        extras["vadv_512x512x160_f64_reference_formula_gbs"] = 10 * points * 8 / vadv_s / 1e9
+        extras["vadv_512x512x160_f64_gridpoints_per_s"] = points / vadv_s
         extras["vadv_256x256x80_f64_gbs"], _ = bench_vadv((256, 256, 80), np.float64,
                                                           native.SB_F64, args.steps)
         extras["vadv_256x256x80_f32s_f64c_gbs"], _ = bench_vadv((256, 256, 80), np.float32,
