@@ -32,6 +32,22 @@ def test_serve_results_alone_equal_mix(gpu, dtype):
         assert np.array_equal(u_mix, u_one) and np.array_equal(ts_mix, ts_one), req
 
 
+def test_serve_under_memory_pressure_keeps_results(gpu):
+    # an HBM budget of the largest request plus 1 MiB: admission waits (head of line)
+    # for retirements, with a tick in flight, and reuses freed arena blocks
+    from stencil_benchmarks_amd import serve
+
+    largest = max(serve.footprint_bytes(r.domain, (3, 3, 3), "float64") for r in _requests())
+    tight = serve.Server(dtype="float64", keep_results=True, hbm_budget=largest + (1 << 20))
+    stats = tight.run(_requests(), realtime=False)
+    assert stats.requests == 7
+    roomy = serve.Server(dtype="float64", keep_results=True)
+    roomy.run(_requests(), realtime=False)
+    assert stats.max_active < 7 and stats.ticks > max(r.steps for r in _requests())
+    for rid, (u, ts) in roomy.results.items():
+        assert np.array_equal(u, tight.results[rid][0]) and np.array_equal(ts, tight.results[rid][1])
+
+
 def test_serve_realtime_arrivals_keep_results(gpu):
     # arrivals spread over time (idle waits, joins at later ticks, a tick in
     # flight while the next is enqueued): same results as the closed loop
