@@ -47,15 +47,16 @@ def _widened(fields):
     return [f.astype(np.float64) for f in fields]
 
 
-@pytest.mark.parametrize("storage", ["float64", "float32"])
-def test_hdiff_512x512x160(gpu, storage):
+@pytest.mark.parametrize("storage,compute", [("float64", "float64"), ("float32", "float64"),
+                                             ("float32", "float32")])
+def test_hdiff_512x512x160(gpu, storage, compute):
     domain = (512, 512, 160)
     dtype = np.dtype(storage)
     inp, coeff, out = make_fields(domain, HALO, dtype, alignment=64, count=3, seed=21)
     out[...] = -1.0
     devs, geom = _device([inp, coeff, out], domain, HALO)
     try:
-        code = gpu.precision_code(dtype, np.float64)
+        code = gpu.precision_code(dtype, compute)
         ptrs = [ctypes.c_void_p(d.pointer()) for d in devs]
         gpu.call("sb_hdiff", code, *ptrs, geom, gpu.launch_options(), None)
         gpu.call("sb_synchronize")
@@ -63,7 +64,7 @@ def test_hdiff_512x512x160(gpu, storage):
     finally:
         for d in devs:
             d.free()
-    if dtype == np.float64:
+    if dtype == np.float64 or compute == "float32":  # the oracle in the storage type
         expected = oracle.hdiff(inp, coeff, domain, HALO)
     else:
         wide_inp, wide_coeff = _widened([inp, coeff])
@@ -111,6 +112,26 @@ def test_vadv_full_size(gpu, domain, storage):
     else:
         expected = oracle.vadv(*_widened(fields), domain, HALO).astype(np.float32)
     assert np.array_equal(got[inner(domain, HALO)], expected)
+
+
+def test_vadv_all_components_256x256x80(gpu):
+    # u, v and w with the wcon neighbour shifts (1, 0), (0, 1) and (0, 0)
+    domain = (256, 256, 80)
+    fields = make_fields(domain, HALO, np.float64, alignment=64, count=13, seed=24)
+    devs, geom = _device(fields, domain, HALO)
+    try:
+        pointers = (ctypes.c_void_p * 16)(*([d.pointer() for d in devs] + [0, 0, 0]))
+        gpu.call("sb_vadv", gpu.SB_F64, 1, pointers, geom, gpu.launch_options(), None)
+        gpu.call("sb_synchronize")
+        got = [_download(devs[3 + 4 * c], fields[3 + 4 * c]) for c in range(3)]
+    finally:
+        for d in devs:
+            d.free()
+    wcon = fields[12]
+    for c, shift in enumerate([(1, 0), (0, 1), (0, 0)]):
+        stage, pos, tens, ts = fields[4 * c:4 * c + 4]
+        expected = oracle.vadv(stage, pos, tens, ts, wcon, domain, HALO, shift)
+        assert np.array_equal(got[c][inner(domain, HALO)], expected), shift
 
 
 @pytest.mark.parametrize("init", ["constant", "random"])
