@@ -221,7 +221,8 @@ int sb_stream_check(int prec, const void *x, size_t n, double expected,
  * per item / non-temporal flags (as sb_hdiff); vadv_launch->variant = 4
  * (default: warp-specialised solver, items of ->by column groups, 0 = 16,
  * dealt to the workgroups by the host in order of decreasing cost, ->unroll
- * = levels per barrier step 4 or 8, 0 = 4) or 3
+ * = levels per barrier step 4 or 8, 0 = 4, ->bx = waves per workgroup 4 or
+ * 5, 0 = 4, the fifth loading wcon for the others) or 3
  * (single-wave pipeline, items from a device counter; ->unroll = prefetch
  * depth 4 or 8, ->bz = 1 non-temporal streamed loads); with variant 4,
  * ->bz bit 2 selects the fast reciprocal (SB_F32_F64 only). */

@@ -7,8 +7,8 @@ options on the device and keeps the fastest per device and workload:
 
 * batched steps (`tune_batch`): hdiff tile height, k-planes per item and
   cache hints, then the vadv solver (warp-specialised with 8/16/32 column
-  groups per item and 4 or 8 levels per barrier step, or the single-wave
-  pipeline at prefetch depth 4/8),
+  groups per item, 4 or 8 levels per barrier step and 4 or 5 waves, or the
+  single-wave pipeline at prefetch depth 4/8),
   chosen by coordinate descent (hdiff with the default vadv, then vadv with
   the chosen hdiff), each candidate timed over whole steps on scratch
   subdomains of the workload's geometry (user fields are never advanced);
@@ -46,11 +46,14 @@ BATCH_HDIFF_SPACE = (
     {"by": 16, "bz": 1}, {"by": 8, "bz": 1},
 )
 # vadv options of the batched step (variant 4: ->by column groups per item,
-# ->unroll levels per barrier step; variant 3: ->unroll prefetch depth)
+# ->unroll levels per barrier step, ->bx waves per workgroup; variant 3:
+# ->unroll prefetch depth)
 BATCH_VADV_SPACE = (
     {"variant": 4, "by": 16}, {"variant": 4, "by": 8}, {"variant": 4, "by": 32},
     {"variant": 4, "by": 16, "unroll": 8}, {"variant": 4, "by": 8, "unroll": 8},
     {"variant": 4, "by": 32, "unroll": 8},
+    {"variant": 4, "by": 8, "bx": 5}, {"variant": 4, "by": 16, "bx": 5},
+    {"variant": 4, "by": 16, "unroll": 8, "bx": 5},
     {"variant": 3, "unroll": 8}, {"variant": 3, "unroll": 4},
 )
 # plain hdiff: tile rows x k-planes x cache hints (->unroll 0: non-temporal

@@ -41,6 +41,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "hdiff_tile.h"
@@ -179,9 +180,10 @@ void vadv_batch_kernel(const SubDev *__restrict__ subs, const int64_t *__restric
 // schedule); w is the logical worker of this workgroup, numbered so that
 // consecutive workers share an XCD: the schedule hands consecutive items of
 // a subdomain (adjacent groups of a row, which share the cache lines at
-// their edges) to consecutive workers.  MB levels per barrier step.
-template <typename T, typename C, bool FAST, int MB>
-__global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_batch_kernel(
+// their edges) to consecutive workers.  MB levels per barrier step, NW
+// waves per workgroup.
+template <typename T, typename C, bool FAST, int MB, int NW>
+__global__ __launch_bounds__(64 * NW) void vadv_ws_batch_kernel(
     const SubDev *__restrict__ subs, const int32_t *__restrict__ off,
     const VItem *__restrict__ list, int xcds) {
     extern __shared__ unsigned char lds_raw[];
@@ -199,8 +201,8 @@ __global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_batch_kernel(
                          uint32_t(xcds)};
         const WsFields f{d.out, d.pos, d.tens, d.ts, d.wcon, d.span_bytes};
         // u component: wcon neighbour at i+1
-        WsSolver<T, C, 0, FAST, MB>::run(f, d.si, a, WsSeq{r, m, (groups - r + m - 1) / m}, lane0,
-                                         lane0 + ws_rows(d.nz, MB) * 64);
+        WsSolver<T, C, 0, FAST, MB, NW>::run(f, d.si, a, WsSeq{r, m, (groups - r + m - 1) / m},
+                                             lane0, lane0 + ws_rows(d.nz, MB) * 64);
     }
 }
 
@@ -262,6 +264,10 @@ int vadv_ws_block(const sb_launch *l) { return (l && l->unroll == 8) ? 8 : kWsBl
 bool vadv_ws_block_ok(const sb_launch *l) {
     return !l || l->unroll == 0 || l->unroll == 4 || l->unroll == 8;
 }
+// waves per workgroup of the warp-specialised solver (launch->bx: 4 or 5,
+// 0 = 4; five add a wave for the wcon loads)
+int vadv_ws_waves(const sb_launch *l) { return (l && l->bx == 5) ? 5 : kWsWaves; }
+bool vadv_ws_waves_ok(const sb_launch *l) { return !l || l->bx == 0 || l->bx == 4 || l->bx == 5; }
 int vadv_pf(const sb_launch *l) { return (l && l->unroll) ? l->unroll : 8; }
 int vadv_nt(const sb_launch *l) { return (l && l->bz) ? (l->bz > 0 ? 1 : 0) : 0; }
 bool vadv_fast(const sb_launch *l) { return l && l->bz > 0 && (l->bz & 4); }
@@ -473,26 +479,29 @@ hipError_t enqueue_step(const Plan &p, int flip, unsigned long long *counters, C
             const unsigned grid = unsigned(c.per_cu * ctx->compute_units);
             // the schedule was built for this grid: one list per workgroup
             if (c.host_off.size() != size_t(grid) + 1) return hipErrorInvalidValue;
-            const int block = vadv_ws_block(&p.vl);
+            const int block = vadv_ws_block(&p.vl), waves = vadv_ws_waves(&p.vl);
             const size_t wlds = ws_lds_bytes<C>(c.max_nz, block);
             auto launch_w = [&](auto kernel) -> hipError_t {
                 (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(wlds));
-                hipLaunchKernelGGL(kernel, dim3(grid), dim3(64 * kWsWaves), wlds, ctx->stream,
+                hipLaunchKernelGGL(kernel, dim3(grid), dim3(64 * waves), wlds, ctx->stream,
                                    subs, c.off, c.list, ctx->xcds);
                 return hipGetLastError();
             };
+            // (fast reciprocal, levels per step, waves) -> kernel
+            auto launch_shape = [&](auto fast_tag) -> hipError_t {
+                constexpr bool F = decltype(fast_tag)::value;
+                if (block == 8)
+                    return waves == 5 ? launch_w(vadv_ws_batch_kernel<T, C, F, 8, 5>)
+                                      : launch_w(vadv_ws_batch_kernel<T, C, F, 8, kWsWaves>);
+                return waves == 5 ? launch_w(vadv_ws_batch_kernel<T, C, F, kWsBlock, 5>)
+                                  : launch_w(vadv_ws_batch_kernel<T, C, F, kWsBlock, kWsWaves>);
+            };
             hipError_t e = hipSuccess;
             if constexpr (sizeof(T) == 4 && sizeof(C) == 8) {
-                if (vadv_fast(&p.vl))
-                    e = block == 8 ? launch_w(vadv_ws_batch_kernel<T, C, true, 8>)
-                                   : launch_w(vadv_ws_batch_kernel<T, C, true, kWsBlock>);
-                else
-                    e = block == 8 ? launch_w(vadv_ws_batch_kernel<T, C, false, 8>)
-                                   : launch_w(vadv_ws_batch_kernel<T, C, false, kWsBlock>);
+                e = vadv_fast(&p.vl) ? launch_shape(std::true_type{}) : launch_shape(std::false_type{});
             } else {
-                e = block == 8 ? launch_w(vadv_ws_batch_kernel<T, C, false, 8>)
-                               : launch_w(vadv_ws_batch_kernel<T, C, false, kWsBlock>);
+                e = launch_shape(std::false_type{});
             }
             if (e != hipSuccess) return e;
         }
@@ -605,6 +614,9 @@ int check_lds(const Plan &p, Context *ctx) {
     SB_REQUIRE(variant != 4 || vadv_ws_block_ok(&p.vl),
                "warp-specialised vadv: levels per step (launch->unroll) must be 4 or 8, got %d",
                p.vl.unroll);
+    SB_REQUIRE(variant != 4 || vadv_ws_waves_ok(&p.vl),
+               "warp-specialised vadv: waves per workgroup (launch->bx) must be 4 or 5, got %d",
+               p.vl.bx);
     const size_t lds = plan_vadv_lds(p.prec, p.max_nz, variant, vadv_ws_block(&p.vl));
     SB_REQUIRE(lds <= ctx->lds_per_block,
                "tallest column nz = %d needs %zu bytes of LDS, device allows %zu", p.max_nz, lds,

@@ -16,6 +16,10 @@
 //                s(k+1) = wcon(i+is, k+1) + wcon(k+1) -> slot.c;
 //   C  (wave 3): the recurrence of group n from slot (e, s) alone:
 //                a, b, c from s(k), s(k+1), then c'(k), d'(k) -> slot.
+// With five waves (a choice of the batched kernel) a fifth wave W takes
+// P2's wcon loads: it writes s(k+1) -> slot.c on P1's steps (the rows P1
+// fills are free in slot.c too), and P2 reads s from the slot; neither the
+// skew nor the barrier count changes.
 // C issues no global memory access: it runs the serial dependence chain
 // only.  P1, P2 and B are independent per level and keep many loads in
 // flight.  A single wave per SIMD issues about one instruction every 7-9
@@ -57,7 +61,7 @@
 namespace sb {
 namespace {
 
-constexpr int kWsWaves = 4;
+constexpr int kWsWaves = 4;  // default waves per workgroup (the other choice: 5, W below)
 constexpr int kWsBlock = 4;  // default levels per step (the other choice: 8)
 constexpr int kWsSkew = 3;   // steps from B's read of a block to C's write
 constexpr int kWsRing = 16;  // prefetch ring of the loading waves, levels
@@ -121,10 +125,14 @@ __device__ __forceinline__ Rsrc ws_rsrc(const WsFields &f) {
                 make_rsrc(f.wcon, f.span_bytes)};
 }
 
-template <typename T, typename C, int AUX, bool FAST, int MB = kWsBlock>
+template <typename T, typename C, int AUX, bool FAST, int MB = kWsBlock, int NW = kWsWaves>
 struct WsSolver {
     static constexpr int M = MB, R = kWsRing, U = kWsRing / MB;
     static_assert(MB == 4 || MB == 8, "levels per step: 4 or 8");
+    static_assert(NW == 4 || NW == 5, "waves per workgroup: 4 or 5");
+    // with five waves the wcon loads move from P2 to a wave of their own (W)
+    static constexpr bool kSplitW = NW == 5;
+    static constexpr int RW = kSplitW ? 1 : R;  // P2's wcon ring (unused when split)
 
     struct Col {  // one group's addressing
         GroupOff g;
@@ -352,18 +360,100 @@ struct WsSolver {
         idle(nb + kWsSkew);  // last iteration: backward only
     }
 
+    // ---- W (five waves only): s(k+1) = wcon(i+is, k+1) + wcon(k+1) -> slot.c ----
+    // at P1's steps (the rows P1 fills with d0 are free in slot.c too), so
+    // P2 reads s from the slot instead of loading wcon
+    struct WState {
+        T rw[R], rwn[R];  // wcon and its neighbour at level k+1, level k at k mod R
+        uint32_t so;      // offset of the next level to load
+    };
+
+    __device__ __forceinline__ static void w_load(const Rsrc &Rs, const Col &c, WState &s, int r,
+                                                  uint32_t so) {
+        s.rw[r] = bload<T, 0>(Rs.w, c.g.lane, so);
+        s.rwn[r] = bload<T, 0>(Rs.w, c.g.lane_wn, so);
+    }
+
+    __device__ __forceinline__ static void w_prologue(const Rsrc &Rs, const Col &c, WState &s) {
+#pragma unroll
+        for (int i = 0; i < R; ++i) w_load(Rs, c, s, i, c.clamped(i + 1));
+        s.so = c.clamped(R + 1);
+    }
+
+    template <bool ODD>
+    __device__ __forceinline__ static void w_blocks(const Rsrc &Rs, const Col &c, WState &s,
+                                                    C *cbuf) {
+        for (int b0 = 0; b0 < c.nb; b0 += U) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int b = b0 + u;
+                C *cb = cbuf + c.template row0<ODD>(b) * 64;
+                C sn[M];
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    const int r = u * M + j;
+                    sn[j] = C(s.rwn[r]) + C(s.rw[r]);
+                }
+                if (b * M + M <= c.nz) {
+#pragma unroll
+                    for (int j = 0; j < M; ++j) cb[in_block<ODD>(j)] = sn[j];
+                } else {
+#pragma unroll
+                    for (int j = 0; j < M; ++j)
+                        if (b * M + j < c.nz) cb[in_block<ODD>(j)] = sn[j];
+                }
+#pragma unroll
+                for (int j = 0; j < M; ++j) {
+                    w_load(Rs, c, s, u * M + j, s.so);
+                    s.so = c.next(s.so);
+                }
+                if (b < c.nb) ws_barrier();
+            }
+        }
+    }
+
+    // W keeps P1's schedule: steps 1 .. nb, next ring in steps nb+1, nb+2.
+    __device__ __forceinline__ static void prep_s(const WsFields &f, const PipeArgs &a,
+                                                  int64_t wshift, const WsSeq &q, C *cbuf) {
+        const Rsrc Rs = ws_rsrc(f);
+        const int nz = int(a.nz), nb = (nz + M - 1) / M;
+        WState s;
+        Col c = column(a, group(q, 0), wshift);
+        w_prologue(Rs, c, s);
+        int64_t n = 0;
+        for (; n + 1 < q.count; n += 2) {
+            idle(1);
+            w_blocks<false>(Rs, c, s, cbuf);
+            c = column(a, group(q, n + 1), wshift);
+            w_prologue(Rs, c, s);
+            idle(kWsSkew);
+            w_blocks<true>(Rs, c, s, cbuf);
+            c = column(a, group(q, n + 2), wshift);
+            w_prologue(Rs, c, s);
+            idle(kWsSkew - 1);
+        }
+        if (n < q.count) {
+            idle(1);
+            w_blocks<false>(Rs, c, s, cbuf);
+            idle(kWsSkew - 1);
+        }
+        idle(nb + kWsSkew);
+    }
+
     // ---- P2: e = d0 + corr, s(k+1) -----------------------------------------------
     struct P2State {
-        T rst[R], rw[R], rwn[R];  // stage, wcon, neighbour at level k+1
+        T rst[R], rw[RW], rwn[RW];  // stage, wcon, neighbour at level k+1
         T st0;
-        uint32_t so;              // offset of the next level to load
+        uint32_t so;                // offset of the next level to load
     };
 
     __device__ __forceinline__ static void p2_load(const Rsrc &Rs, const Col &c, P2State &s, int r,
                                                    uint32_t so) {
         s.rst[r] = bload<T, AUX>(Rs.stage, c.g.lane, so);
-        s.rw[r] = bload<T, 0>(Rs.w, c.g.lane, so);
-        s.rwn[r] = bload<T, 0>(Rs.w, c.g.lane_wn, so);
+        if constexpr (!kSplitW) {
+            s.rw[r] = bload<T, 0>(Rs.w, c.g.lane, so);
+            s.rwn[r] = bload<T, 0>(Rs.w, c.g.lane_wn, so);
+        }
     }
 
     __device__ __forceinline__ static void p2_prologue(const Rsrc &Rs, const Col &c, P2State &s) {
@@ -390,18 +480,25 @@ struct WsSolver {
                 C *cb = cbuf + rb;
                 C *db = dbuf + rb;
                 if (b >= 1 && b * M + M <= c.nz - 1) {
-                    C d0[M];  // block's slot reads first (latency overlaps)
+                    C d0[M], sv[M];  // block's slot reads first (latency overlaps)
 #pragma unroll
-                    for (int j = 0; j < M; ++j) d0[j] = db[in_block<ODD>(j)];
+                    for (int j = 0; j < M; ++j) {
+                        d0[j] = db[in_block<ODD>(j)];
+                        if constexpr (kSplitW) sv[j] = cb[in_block<ODD>(j)];
+                    }
 #pragma unroll
                     for (int j = 0; j < M; ++j) {
                         const int r = u * M + j, o = in_block<ODD>(j);
                         const C st_n = C(s.rst[r]);
-                        const C s_n = C(s.rwn[r]) + C(s.rw[r]);
+                        C s_n;
+                        if constexpr (kSplitW)
+                            s_n = sv[j];
+                        else
+                            s_n = C(s.rwn[r]) + C(s.rw[r]);
                         const C cs = (C(0.25) * s_n) * bet_m;
                         const C corr = cs_p * (st_m - st_c) - cs * (st_n - st_c);
                         db[o] = d0[j] + corr;  // e = d0 + corr
-                        cb[o] = s_n;
+                        if constexpr (!kSplitW) cb[o] = s_n;
                         st_m = st_c;
                         st_c = st_n;
                         cs_p = cs;
@@ -412,9 +509,13 @@ struct WsSolver {
                         const int r = u * M + j, o = in_block<ODD>(j);
                         const int k = b * M + j;
                         const C st_n = C(s.rst[r]);
-                        const C s_n = C(s.rwn[r]) + C(s.rw[r]);
-                        const C cs = (C(0.25) * s_n) * bet_m;
-                        if (k < c.nz) {
+                        if (k < c.nz) {  // slot rows exist for levels below nz only
+                            C s_n;
+                            if constexpr (kSplitW)
+                                s_n = cb[o];
+                            else
+                                s_n = C(s.rwn[r]) + C(s.rw[r]);
+                            const C cs = (C(0.25) * s_n) * bet_m;
                             C corr;
                             if (k == 0)
                                 corr = -cs * (st_n - st_c);
@@ -423,11 +524,11 @@ struct WsSolver {
                             else
                                 corr = cs_p * (st_m - st_c);
                             db[o] = db[o] + corr;
-                            cb[o] = s_n;
+                            if constexpr (!kSplitW) cb[o] = s_n;
+                            cs_p = cs;
                         }
                         st_m = st_c;
                         st_c = st_n;
-                        cs_p = cs;
                     }
                 }
 #pragma unroll
@@ -569,7 +670,7 @@ struct WsSolver {
         idle(nb + kWsSkew);
     }
 
-    // All four waves of the workgroup call this with the same arguments.
+    // All NW waves of the workgroup call this with the same arguments.
     __device__ __forceinline__ static void run(const WsFields &f, int64_t wshift, const PipeArgs &a,
                                                const WsSeq &q, C *cbuf, C *dbuf) {
         const int role = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
@@ -579,8 +680,10 @@ struct WsSolver {
             prep_d0(f, a, q, dbuf);
         else if (role == 2)
             prep_e(f, a, wshift, q, cbuf, dbuf);
-        else
+        else if (role == 3)
             chain(a, q, cbuf, dbuf);
+        else if constexpr (kSplitW)
+            prep_s(f, a, wshift, q, cbuf);
     }
 };
 

@@ -59,6 +59,7 @@ def test_spaces_hold_only_built_combinations():
     assert len(autotune.PLAIN_HDIFF_SPACE) == 3 * 5 * 4
     for v in autotune.BATCH_VADV_SPACE:  # levels per step 4 / 8, prefetch depth 4 / 8
         assert v["variant"] in (3, 4) and v.get("unroll", 0) in (0, 4, 8)
+        assert v.get("bx", 0) in (0, 4, 5) and (v["variant"] == 4 or "bx" not in v)
 
 
 def test_launch_from_sets_fields():
