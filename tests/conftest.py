@@ -14,8 +14,8 @@ if str(ROOT) not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device 0)")
     # the CPU oracle (test infrastructure) is built here when missing, so the
-    # CPU suite does not depend on a prior build step; on the GPU box it is
-    # prebuilt and travels with the tree
+    # CPU suite does not depend on a prior build step; where it already
+    # exists (built by build()), it is used as is
     oracle_lib = ROOT / "oracle" / "liboracle.so"
     if not oracle_lib.exists() and shutil.which("gcc"):
         subprocess.run(["make", "-C", str(ROOT / "oracle"), "liboracle.so"], check=False,
