@@ -134,12 +134,18 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
     for (int r = 0; r < queues; ++r) {
         const int q = home + r < queues ? home + r : home + r - queues;
         const int64_t lo = total * q / queues, hi = total * (q + 1) / queues;
+        // the next item's ticket is drawn while the current item runs, so the
+        // atomic's round trip overlaps the tile's loads; every ticket drawn
+        // is either worked or ends this range
+        unsigned long long next = 0;
+        if (tid == 0) next = atomicAdd(counters + q, 1ull);
         for (;;) {
-            if (tid == 0) s_item = lo + (long long)atomicAdd(counters + q, 1ull);
+            if (tid == 0) s_item = lo + (long long)next;
             __syncthreads();
             const int64_t item = uniform64(uint64_t(s_item));
             __syncthreads();
             if (item >= hi) break;
+            if (tid == 0) next = atomicAdd(counters + q, 1ull);
             const int s = find_sub(prefix, count, item);
             const SubDev d = subs[s];
             const int64_t local = item - prefix[s];
