@@ -131,6 +131,10 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
     const int64_t total = prefix[count];
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const int home = int(blockIdx.x % uint32_t(queues));
+    // subdomain of the last item and its item range: a workgroup's tickets
+    // mostly stay in one subdomain, which then needs no search
+    int sub = 0;
+    int64_t sub_lo = 0, sub_hi = 0;
     for (int r = 0; r < queues; ++r) {
         const int q = home + r < queues ? home + r : home + r - queues;
         const int64_t lo = total * q / queues, hi = total * (q + 1) / queues;
@@ -146,9 +150,13 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
             __syncthreads();
             if (item >= hi) break;
             if (tid == 0) next = atomicAdd(counters + q, 1ull);
-            const int s = find_sub(prefix, count, item);
-            const SubDev d = subs[s];
-            const int64_t local = item - prefix[s];
+            if (item < sub_lo || item >= sub_hi) {
+                sub = find_sub(prefix, count, item);
+                sub_lo = prefix[sub];
+                sub_hi = prefix[sub + 1];
+            }
+            const SubDev d = subs[sub];
+            const int64_t local = item - sub_lo;
             const int64_t tiles = int64_t(d.tiles_i) * d.tiles_j;
             const int64_t kc = local / tiles;
             const int64_t t = local - kc * tiles;
