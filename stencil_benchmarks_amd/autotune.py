@@ -163,7 +163,7 @@ class TuneResult:
         return launch_from(self.vadv)
 
 
-def tune_batch(specs, storage, compute=None, cache=None, repeats=2, force=False):
+def tune_batch(specs, storage, compute=None, cache=None, repeats=3, force=False, warmup=4):
     """Tuned launch options for batched steps over subdomains of `specs`
     (objects with .domain, .halo) in `storage` precision with `compute`
     arithmetic; times whole steps on scratch subdomains of the same shapes."""
@@ -181,6 +181,12 @@ def tune_batch(specs, storage, compute=None, cache=None, repeats=2, force=False)
     scratch = [batch.Subdomain(s.domain, s.halo, storage, seed=index)
                for index, s in enumerate(specs)]
     try:
+        # untimed steps first: the first launches on fresh fields and a cold
+        # device run slower and would count against the first candidate
+        for _ in range(warmup):
+            batch.step(scratch, compute=compute)
+        native.call("sb_synchronize")
+
         def measure(h, v):
             hl, vl = launch_from(h), launch_from(v)
             batch.step(scratch, compute=compute, hdiff_options=hl, vadv_options=vl)  # warm-up
@@ -197,7 +203,7 @@ def tune_batch(specs, storage, compute=None, cache=None, repeats=2, force=False)
 
 
 def tune_hdiff(domain, storage, compute=None, halo=(3, 3, 3), cache=None, repeats=3,
-               force=False):
+               force=False, warmup=20):
     """Tuned sb_hdiff launch options for one domain (cold-cache launches)."""
     import ctypes
 
@@ -235,6 +241,11 @@ def tune_hdiff(domain, storage, compute=None, halo=(3, 3, 3), cache=None, repeat
                 runs.append(seconds.value)
             return min(runs)
 
+        # untimed, with the library's default launch (it takes any domain): a
+        # cold device would count against the first candidate
+        for _ in range(warmup):
+            native.call("sb_hdiff", code, *ptrs, geom, native.launch_options(), None)
+
         h, _, seconds, trials = descend(PLAIN_HDIFF_SPACE, (), measure, {})
     finally:
         fields[0].free()
@@ -243,7 +254,7 @@ def tune_hdiff(domain, storage, compute=None, halo=(3, 3, 3), cache=None, repeat
 
 
 def tune_vadv(domain, storage, compute=None, halo=(3, 3, 3), cache=None, repeats=3,
-              force=False):
+              force=False, warmup=20):
     """Tuned sb_vadv launch options (u component) for one domain (cold-cache
     launches).  Solvers that the domain does not fit (LDS, 4 GiB buffer
     offsets) are skipped."""
@@ -287,6 +298,10 @@ def tune_vadv(domain, storage, compute=None, halo=(3, 3, 3), cache=None, repeats
                 return float("inf")
             return min(runs)
 
+        # untimed, with the library's default launch (it takes any domain): a
+        # cold device would count against the first candidate
+        for _ in range(warmup):
+            native.call("sb_vadv", code, 0, pointers, geom, native.launch_options(), None)
         _, v, seconds, trials = descend(({},), PLAIN_VADV_SPACE, measure, PLAIN_VADV_SPACE[0])
     finally:
         fields[0].free()
