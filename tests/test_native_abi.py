@@ -65,3 +65,14 @@ def test_missing_library_fails_loudly(tmp_path):
 
     with pytest.raises(native.NativeLibraryMissing):
         native.load(tmp_path / "libsbmi355x.so")
+
+
+def test_event_entry_points_reject_null_without_a_device(library):
+    # argument checks come before any device call
+    from stencil_benchmarks_amd import native
+
+    lib = native.load(LIBRARY)
+    assert lib.sb_event_record(None) != 0 and b"NULL" in lib.sb_last_error()
+    assert lib.sb_event_synchronize(None) != 0 and b"NULL" in lib.sb_last_error()
+    assert lib.sb_event_create(None) != 0 and b"NULL" in lib.sb_last_error()
+    assert lib.sb_event_destroy(None) == 0
