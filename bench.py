@@ -376,8 +376,8 @@ def main():
     # busy-waiting the GPU beforehand does not remove it)
     parser.add_argument("--warmup", type=int, default=60)
     parser.add_argument("--quick", action="store_true", help="headline only")
-    parser.add_argument("--only", choices=["hdiff", "vadv", "laplacian", "stream", "batched",
-                                           "serving"],
+    parser.add_argument("--only", choices=["hdiff", "hdiff_f32", "vadv", "laplacian", "stream",
+                                           "batched", "serving"],
                         help="run a single workload (profiling)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = parser.parse_args()
@@ -508,6 +508,12 @@ def run_single(args):
             (512, 512, 160), np.float64, native.SB_F64, args.steps)
         out["vadv_512x512x160_f32s_f64c_gbs"], out["mean_s_f32s"] = bench_vadv(
             (512, 512, 160), np.float32, native.SB_F32_F64, args.steps)
+        out["vadv_256x256x80_f64_gbs"], out["mean_s_256"] = bench_vadv(
+            (256, 256, 80), np.float64, native.SB_F64, args.steps)
+        out["vadv_256x256x80_f32s_f64c_gbs"], out["mean_s_256_f32s"] = bench_vadv(
+            (256, 256, 80), np.float32, native.SB_F32_F64, args.steps)
+    elif args.only == "hdiff_f32":
+        out["hdiff_512x512x160_f32s_f64c_gbs"], out["mean_s"] = bench_hdiff_f32(args.steps)
     elif args.only == "laplacian":
         out["laplacian_gbs"], out["mean_s"] = bench_lap(args.steps)
     elif args.only == "stream":

@@ -27,12 +27,20 @@ import copy
 
 import numpy as np
 
-from ...benchmark import Benchmark, Parameter, ParameterError
+from ...benchmark import Benchmark, ExecutionError, Parameter, ParameterError
 from ...tools import array, fill, validation
 
 DTR_STAGE = 3 / 20
 BET_M = 0.5 * (1 - 0)
 BET_P = 0.5 * (1 + 0)
+
+
+def _names_numpy_type(name):
+    try:
+        np.dtype(name)
+    except TypeError:
+        return False
+    return True
 
 
 class Stencil(Benchmark):
@@ -55,41 +63,58 @@ class Stencil(Benchmark):
 
     oracle_dtype = None
 
+    # (test, complaint) pairs over the parameter values; setup raises
+    # ParameterError with the complaint of the first failing test
+    _CONFIGURATION_RULES = (
+        (lambda s: _names_numpy_type(s.dtype), "dtype {s.dtype!r} is not a NumPy data type"),
+        (lambda s: min(s.halo) >= 0, "halo widths must not be negative: {s.halo}"),
+        (lambda s: min(s.domain) >= 0, "domain extents must not be negative: {s.domain}"),
+        (lambda s: sorted(s.layout) == [0, 1, 2],
+         "layout {s.layout} is not a permutation of (0, 1, 2)"),
+        (lambda s: s.data_sets >= 1, "at least one data set is needed, got {s.data_sets}"),
+        (lambda s: s.alignment >= 0, "alignment must not be negative: {s.alignment} bytes"),
+        (lambda s: s.alignment % s.dtype_size == 0,
+         "alignment of {s.alignment} bytes is not a multiple of the {s.dtype_size}-byte "
+         "element size"),
+    )
+
     def setup(self):
         super().setup()
-        if any(h < 0 for h in self.halo):
-            raise ParameterError(f"negative halo size given ({self.halo}")
-        if tuple(sorted(self.layout)) != (0, 1, 2):
-            raise ParameterError(f"invalid layout specification {self.layout}")
-        if self.alignment < 0:
-            raise ParameterError(f"negative alignment given ({self.alignment} bytes)")
-        if self.alignment % self.dtype_size != 0:
-            raise ParameterError(
-                f"alignment ({self.alignment} bytes) not divisible "
-                f"by dtype size ({self.dtype_size} bytes)"
-            )
-        if any(d < 0 for d in self.domain):
-            raise ParameterError(f"negative domain size given ({self.domain})")
+        for test, complaint in self._CONFIGURATION_RULES:
+            if not test(self):
+                raise ParameterError(complaint.format(s=self))
+        self._data = [self._new_data_set(number) for number in range(self.data_sets)]
+        self._runs_started = 0
+
+    def _new_data_set(self, number):
+        """Every argument of data set `number`: outputs zero, the other
+        fields U[0,1) from the generator seeded (seed, number, argument)."""
         record = collections.namedtuple("StencilData", self.args)
-        self._data = []
-        for data_set in range(self.data_sets):
-            fields = []
-            for index, name in enumerate(self.args):
-                field = self.empty_field()
-                if name in self.outputs:
-                    field[...] = 0
-                else:
-                    fill.random_fill(field, fill.field_rng(self.seed, data_set, index))
-                fields.append(field)
-            self._data.append(record._make(fields))
-        self._run = 0
+        fields = []
+        for position, name in enumerate(self.args):
+            field = self.empty_field()
+            if name in self.outputs:
+                field.fill(0)
+            else:
+                fill.random_fill(field, fill.field_rng(self.seed, number, position))
+            fields.append(field)
+        return record(*fields)
+
+    @property
+    def data_set_index(self):
+        """The data set of the current (or next) run; runs cycle through the
+        sets."""
+        return self._runs_started % self.data_sets
 
     # ---- allocation / geometry -------------------------------------------
     def alloc_field(self, domain_with_halo, layout, index_to_align):
-        def allocate(nbytes):
-            if self.huge_pages == "none":
-                return array.alloc_smallpages(nbytes)
-            return array.alloc_hugepages(nbytes, self.huge_pages == "transparent")
+        if self.huge_pages == "none":
+            allocate = array.alloc_smallpages
+        else:
+            transparent = self.huge_pages == "transparent"
+
+            def allocate(nbytes):
+                return array.alloc_hugepages(nbytes, transparent)
 
         return array.alloc_array(
             domain_with_halo,
@@ -118,26 +143,26 @@ class Stencil(Benchmark):
 
     @property
     def domain_with_halo(self):
-        return tuple(d + 2 * h for d, h in zip(self.domain, self.halo))
+        return tuple(int(n) for n in np.add(self.domain, np.multiply(2, self.halo)))
 
     @property
     def strides(self):
-        return tuple(s // self.dtype_size for s in self._data[0][0].strides)
+        first_field = self._data[0][0]
+        return tuple(s // self.dtype_size for s in first_field.strides)
 
     @property
     def data_size(self):
         return len(self.args) * int(np.prod(self.domain)) * self.dtype_size
 
     def inner_slice(self, shift=None, expand=None):
-        shift = [0] * len(self.domain) if shift is None else shift
-        if expand is None:
-            expand = [0] * len(self.domain)
-        elif isinstance(expand, int):
-            expand = [expand] * len(self.domain)
-        return tuple(
-            slice(h + s - e, h + d + s + e)
-            for d, h, s, e in zip(self.domain, self.halo, shift, expand)
-        )
+        """Slices of the interior, moved by `shift` and widened by `expand`
+        (an int for every axis, or one value per axis) on both sides."""
+        rank = len(self.domain)
+        shift = np.zeros(rank, dtype=int) if shift is None else np.asarray(shift, dtype=int)
+        grow = np.broadcast_to(np.asarray(0 if expand is None else expand, dtype=int), (rank,))
+        first = np.asarray(self.halo) + shift - grow
+        last = first + np.asarray(self.domain) + 2 * grow
+        return tuple(slice(int(a), int(b)) for a, b in zip(first, last))
 
     # ---- lifecycle ------------------------------------------------------------
     @abc.abstractmethod
@@ -154,17 +179,21 @@ class Stencil(Benchmark):
         pass
 
     def run(self):
-        index = self._run % self.data_sets
-        if self.verify:
-            before = copy.deepcopy(self._data[index])
-        result = self.run_stencil(self._data[index])
-        if self.verify:
-            self.verify_stencil(before, self._data[index])
-        self._run += 1
-        assert "time" in result and result["time"] > 0
-        assert "bandwidth" not in result
-        result["bandwidth"] = self.data_size / result["time"] / 1e9
-        return result
+        """One execution on the next data set: the kernel, its verification
+        against the oracle on a snapshot of the inputs, and the reference
+        bandwidth figure (data_size / time, GB/s)."""
+        data = self._data[self.data_set_index]
+        snapshot = copy.deepcopy(data) if self.verify else None
+        result = self.run_stencil(data)
+        self._runs_started += 1
+        if snapshot is not None:
+            self.verify_stencil(snapshot, data)
+        if result.get("time", 0) <= 0:
+            raise ExecutionError(f"{type(self).__name__}: run reported no positive time")
+        if "bandwidth" in result:
+            raise ExecutionError(f"{type(self).__name__}: run_stencil must leave "
+                                 "'bandwidth' to the base class")
+        return dict(result, bandwidth=self.data_size / result["time"] / 1e9)
 
     # ---- oracle helpers ---------------------------------------------------
     def _oracle_input(self, field):

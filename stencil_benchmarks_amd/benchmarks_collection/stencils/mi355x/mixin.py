@@ -88,8 +88,7 @@ class StencilMixin(Benchmark):
                                      flush=self.flush_caches)
 
     def run_stencil(self, data):
-        index = self._run % self.data_sets
-        device_set = self._device_data[index]
+        device_set = self._device_data[self.data_set_index]
         pointers = [ctypes.c_void_p(f.pointer()) for f in device_set]
         options = self.launch_options()
         if self.timers == "gpu":

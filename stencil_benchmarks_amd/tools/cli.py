@@ -1,276 +1,287 @@
-"""Click helpers: range-valued options, range unpacking, result columns.
+"""Command-line helpers: options that take value ranges, their expansion into
+runs, the per-run parameter columns and a progress line.
 
-Range grammar of stencil_benchmarks/tools/cli.py:231-416:
+The option grammar and the column names are the reference's interface
+(stencil_benchmarks/tools/cli.py:231-493), so command lines and CSV files
+carry over unchanged:
 
-* ``[a-b]`` / ``[a-b:op s]`` with op one of ``+ - * /`` (default ``+1``);
-* ``[x,y,z]`` explicit values;
-* ``[name=...]`` names a range; ranges sharing a name are zipped, while
-  distinct ranges form a cartesian product; ``[name=]`` reuses the values
-  defined elsewhere under that name.
+* a plain value, or a bracketed range: ``[1,2,5]`` (a list), ``[1-16]``
+  (``start-stop``, step +1) or ``[1-16:*2]`` (``start-stop:<op><step>`` with
+  op one of ``+ - * /``; the values run from start while they stay inside
+  ``[start, stop]``);
+* ``[name=...]`` names a range.  Options that use the same name take their
+  values together (zipped); different ranges multiply (cartesian product);
+  ``[name=]`` uses a range whose values another option defines;
+* result columns: the parameter name with ``_`` replaced by ``-``, tuple
+  values split into ``<name>-0``, ``<name>-1``, ..., plus ``sbench-version``
+  and ``benchmark-name`` (module path and class name).
 
-Result columns follow ``pretty_parameters`` (``cli.py:448-493``): ``_`` in
-names becomes ``-``, tuple values expand to ``name-0``, ``name-1``, ...,
-plus ``sbench-version`` and ``benchmark-name``.
-
->>> unpack_ranges(a=3, b=ArgRange(name='r0', values=(1, 2, 3)))
-[{'a': 3, 'b': 1}, {'a': 3, 'b': 2}, {'a': 3, 'b': 3}]
->>> unpack_ranges(a=ArgRange(name='r0', values=(1, 2)),
-...               b=ArgRange(name='r0', values=None))
-[{'a': 1, 'b': 1}, {'a': 2, 'b': 2}]
+>>> kind = range_type(int)
+>>> kind.convert("[n=2-32:*2]", None, None)
+ArgRange(name='n', values=(2, 4, 8, 16, 32))
+>>> runs = unpack_ranges(size=(kind.convert("[s=8,16]", None, None), 4),
+...                      depth=kind.convert("[s=]", None, None))
+>>> [(r["size"], r["depth"]) for r in runs]
+[((8, 4), 8), ((16, 4), 16)]
 """
 
 import itertools
-import operator
 import re
 import sys
-import types
-from typing import Any, Dict, Iterator, List, NamedTuple, Optional
+from typing import Any, Dict, Iterable, Iterator, List, NamedTuple, Optional
 
 import click
 
 from .. import __version__
 
 
-def colorize(string, rgb=None, hsv=None):
-    """Wrap `string` in a 24-bit terminal colour escape.
-
-    >>> colorize('foo', rgb=(1, 0, 0)) # doctest: +ELLIPSIS
-    '...foo...'
-    """
-    if hsv:
-        hue, sat, val = hsv
-
-        def channel(n):
-            k = (n + hue * 6) % 6
-            return val - val * sat * max(min(k, 4 - k, 1), 0)
-
-        rgb = channel(5), channel(3), channel(1)
-    if not rgb:
-        raise ValueError("missing color specification")
-    red, green, blue = (int(255 * c) for c in rgb)
-    return f"\x1b[38;2;{red};{green};{blue}m{string}\x1b[0m"
-
-
-class ProgressBar:
-    """Nested terminal progress bar.
-
-    >>> ProgressBar().progress
-    0.0
-    """
-
-    def __init__(self):
-        self._levels = []
-
-    def report(self, iterable):
-        items = list(iterable)
-        depth = len(self._levels)
-        self._levels.append(types.SimpleNamespace(current=0, max=len(items)))
-        self._print()
-        try:
-            for item in items:
-                yield item
-                self._levels[depth].current += 1
-                self._print()
-        finally:
-            self._levels.pop()
-
-    @property
-    def progress(self) -> float:
-        percent, width = 0.0, 100.0
-        for level in self._levels:
-            if level.max:
-                percent += level.current / level.max * width
-                width /= level.max
-        return percent
-
-    def __enter__(self):
-        return self
-
-    def __exit__(self, exc_type, exc_val, exc_tb):
-        if exc_type is None:
-            click.echo("\r" + " " * 110 + "\r", nl=False)
-        else:
-            click.echo()
-        return False
-
-    def _print(self):
-        percent = round(self.progress)
-        bar = colorize("█" * percent + "-" * (100 - percent), hsv=(percent / 300, 1, 1))
-        click.echo("\r|" + bar + f"| {percent:3}%", nl=False)
-        sys.stdout.flush()
-
-
 class ArgRange(NamedTuple):
-    """A named range of option values (``values`` None: defined elsewhere)."""
+    """A named set of values for an option; ``values`` None means the values
+    are defined by another option using the same name."""
 
     name: str
     values: Optional[tuple]
 
     def unique_values(self, *others) -> tuple:
-        """Single definition of this range's values among its occurrences.
+        """The one definition of this range among all its uses.
 
-        >>> ArgRange('foo', (1, 2)).unique_values(ArgRange('foo', None))
+        >>> ArgRange("r", (1, 2)).unique_values(ArgRange("r", None))
         (1, 2)
+        >>> ArgRange("r", None).unique_values()
+        Traceback (most recent call last):
+            ...
+        ValueError: range 'r' is used but its values are never given
         """
-        definitions = {r.values for r in (self,) + others if r.values}
-        if not definitions:
-            raise ValueError(f'missing value definition for argument range "{self.name}"')
-        if len(definitions) > 1:
-            raise ValueError(f'multiple definitions for argument range "{self.name}"')
-        return definitions.pop()
+        defined = {use.values for use in (self, *others) if use.values}
+        if len(defined) == 1:
+            return defined.pop()
+        if not defined:
+            raise ValueError(f"range {self.name!r} is used but its values are never given")
+        raise ValueError(f"range {self.name!r} is given different values: "
+                         + " vs ".join(map(str, sorted(defined, key=str))))
 
 
-_RANGE_RE = re.compile(
-    r"^(?P<start>[+-]?\d+)-(?P<stop>[+-]?\d+)(:(?P<op>[*/+-])(?P<step>[+-]?\d+))?$"
-)
-_OPS = {"+": operator.add, "-": operator.sub, "*": operator.mul, "/": operator.truediv}
+# ``start-stop`` with optional signs, then an optional ``:<op><step>``
+_PROGRESSION = re.compile(r"([+-]?\d+)-([+-]?\d+)(?::([-+*/])([+-]?\d+))?")
+_MAX_RANGE_VALUES = 1 << 16
 
 
-class _MaybeRange(click.ParamType):
-    """Click type accepting a plain value or a bracketed range."""
+def _progression(start, stop, op, step):
+    """Values start, start op step, ... while inside [start, stop]."""
+    advance = {"+": lambda v: v + step, "-": lambda v: v - step,
+               "*": lambda v: v * step, "/": lambda v: v / step}[op]
+    values = []
+    value = start
+    while start <= value <= stop:
+        if len(values) == _MAX_RANGE_VALUES or (values and value == values[-1]):
+            raise click.BadParameter(f"range from {start} to {stop} does not progress "
+                                     f"with {op}{step}")
+        values.append(value)
+        value = advance(value)
+    return tuple(values)
 
-    name = "maybe range"
 
-    def __init__(self, base_type):
-        self.base_type = base_type
-        self._anonymous = 0
+class _RangeAware(click.ParamType):
+    """A click type that reads a plain value of `element`, or a range."""
 
-    def _interval(self, text, param, ctx):
-        match = _RANGE_RE.match(text)
-        if not match:
-            self.fail(f'could not parse range "{text}"')
-        start = self.base_type.convert(match.group("start"), param, ctx)
-        stop = self.base_type.convert(match.group("stop"), param, ctx)
-        step_op = _OPS[match.group("op")] if match.group("op") else operator.add
-        step = (
-            self.base_type.convert(match.group("step"), param, ctx)
-            if match.group("step")
-            else 1
-        )
-        values, current = [], start
-        while start <= current <= stop:
-            values.append(current)
-            current = step_op(current, step)
-        return tuple(values)
+    name = "value or range"
+    _unnamed = itertools.count()
+
+    def __init__(self, element: click.ParamType):
+        self.element = element
+
+    def _scalar(self, text, param, ctx):
+        return self.element.convert(text, param, ctx)
 
     def convert(self, value, param, ctx):
-        if isinstance(value, str) and len(value) >= 2 and value[0] == "[" and value[-1] == "]":
-            body = value[1:-1]
-            if "=" in body:
-                name, body = body.split("=", 1)
-            else:
-                name = f"{id(self)}_{self._anonymous}"
-                self._anonymous += 1
-            if not body:
-                values = None
-            elif "," in body:
-                values = tuple(
-                    self.base_type.convert(v, param, ctx) for v in body.split(",")
-                )
-            else:
-                values = self._interval(body, param, ctx)
-            return ArgRange(name, values)
-        return self.base_type.convert(value, param, ctx)
+        if not (isinstance(value, str) and value.startswith("[") and value.endswith("]")
+                and len(value) >= 2):
+            return self._scalar(value, param, ctx)
+        body = value[1:-1]
+        name, equals, spec = body.partition("=")
+        if not equals:
+            name, spec = f"~{next(self._unnamed)}", body
+        if not spec:
+            return ArgRange(name, None)
+        if "," in spec:
+            return ArgRange(name, tuple(self._scalar(v, param, ctx) for v in spec.split(",")))
+        match = _PROGRESSION.fullmatch(spec)
+        if match is None:
+            self.fail(f"cannot read {value!r} as a range: expected [a,b,...], [start-stop] "
+                      "or [start-stop:<op><step>]", param, ctx)
+        start, stop = (self._scalar(match.group(g), param, ctx) for g in (1, 2))
+        op = match.group(3) or "+"
+        step = self._scalar(match.group(4), param, ctx) if match.group(4) else 1
+        return ArgRange(name, _progression(start, stop, op, step))
 
     def get_metavar(self, param, *args, **kwargs):
         try:
-            base = self.base_type.get_metavar(param, *args, **kwargs)
-        except TypeError:
-            base = self.base_type.get_metavar(param)
-        base = base or self.base_type.name
-        return f"{base} [or {base} range]"
+            inner = self.element.get_metavar(param, *args, **kwargs)
+        except TypeError:  # click < 8.2 takes the parameter only
+            inner = self.element.get_metavar(param)
+        inner = inner or self.element.name.upper()
+        return f"{inner}|[RANGE]"
 
 
-_RANGE_TYPES = {
-    int: _MaybeRange(click.INT),
-    float: _MaybeRange(click.FLOAT),
-    str: _MaybeRange(click.STRING),
-}
+_BUILTIN = {int: click.INT, float: click.FLOAT, str: click.STRING}
+_TYPES: Dict[Any, _RangeAware] = {}
 
 
-def range_type(dtype):
-    """Click parameter type for `dtype` that also accepts ranges.
+def range_type(dtype) -> click.ParamType:
+    """Click type of `dtype` (int, float, str or a click type such as
+    ``click.Choice``) that also accepts the range syntax.
 
     >>> @click.command()
-    ... @click.argument('arg', type=range_type(int))
-    ... @click.option('--option', type=range_type(int))
-    ... def foo(arg, option):
-    ...     return arg, option
-    >>> foo(args=['1', '--option', '3'], standalone_mode=False)
-    (1, 3)
-    >>> foo(args=['[x=1-3]', '--option', '[3,4]'], standalone_mode=False)
-    (ArgRange(name='x', values=(1, 2, 3)), ArgRange(name='...', values=(3, 4)))
+    ... @click.option("--n", type=range_type(int))
+    ... def show(n):
+    ...     return n
+    >>> show(["--n", "3"], standalone_mode=False)
+    3
+    >>> show(["--n", "[1,3]"], standalone_mode=False)  # doctest: +ELLIPSIS
+    ArgRange(name='~...', values=(1, 3))
     """
     if isinstance(dtype, click.ParamType):
-        return _MaybeRange(dtype)
-    return _RANGE_TYPES[dtype]
+        return _RangeAware(dtype)
+    if dtype not in _TYPES:
+        _TYPES[dtype] = _RangeAware(_BUILTIN[dtype])
+    return _TYPES[dtype]
 
 
-def _ranges_in(values) -> Iterator[ArgRange]:
-    for value in values:
-        if isinstance(value, ArgRange):
-            yield value
-        elif isinstance(value, tuple):
-            yield from _ranges_in(value)
-
-
-def _substitute(value, chosen):
+def _each_range(value) -> Iterator[ArgRange]:
     if isinstance(value, ArgRange):
-        return chosen[value.name]
+        yield value
+    elif isinstance(value, tuple):
+        for item in value:
+            yield from _each_range(item)
+
+
+def _resolve(value, choice: Dict[str, Any]):
+    if isinstance(value, ArgRange):
+        return choice[value.name]
     if isinstance(value, tuple):
-        return tuple(_substitute(v, chosen) for v in value)
+        return tuple(_resolve(item, choice) for item in value)
     return value
 
 
 def unpack_ranges(**kwargs: Any) -> List[Dict[str, Any]]:
-    """Expand all ArgRanges in `kwargs` into a list of plain kwargs dicts."""
-    by_name = {}
-    for rng in _ranges_in(kwargs.values()):
-        by_name.setdefault(rng.name, []).append(rng)
-    names = sorted(by_name)
-    domains = [ArgRange.unique_values(*by_name[n]) for n in names]
-    return [
-        {key: _substitute(val, dict(zip(names, combo))) for key, val in kwargs.items()}
-        for combo in itertools.product(*domains)
-    ]
+    """One kwargs dict per combination of range values (the range met first
+    varies slowest); kwargs without ranges give a single dict.
+
+    >>> unpack_ranges(a=3, b=5)
+    [{'a': 3, 'b': 5}]
+    >>> unpack_ranges(a=ArgRange("p", (1, 2)), b=ArgRange("q", (3, 4)))
+    [{'a': 1, 'b': 3}, {'a': 1, 'b': 4}, {'a': 2, 'b': 3}, {'a': 2, 'b': 4}]
+    """
+    uses: Dict[str, List[ArgRange]] = {}
+    for value in kwargs.values():
+        for found in _each_range(value):
+            uses.setdefault(found.name, []).append(found)
+    axes = {name: found[0].unique_values(*found[1:]) for name, found in uses.items()}
+    runs = []
+    for combination in itertools.product(*axes.values()):
+        choice = dict(zip(axes, combination))
+        runs.append({key: _resolve(value, choice) for key, value in kwargs.items()})
+    return runs
 
 
 def range_args(**kwargs: Any) -> Iterator[str]:
-    """Names of the arguments in `kwargs` that hold ranges.
+    """Names of the options in `kwargs` that hold a range.
 
-    >>> list(range_args(a=1, b=ArgRange(name='r0', values=(1, 2))))
-    ['b']
+    >>> list(range_args(a=1, b=ArgRange("r", (1, 2)), c=(1, ArgRange("r", None))))
+    ['b', 'c']
     """
-    for key, value in kwargs.items():
-        if isinstance(value, ArgRange) or (
-            isinstance(value, tuple) and any(isinstance(v, ArgRange) for v in value)
-        ):
-            yield key
+    return (key for key, value in kwargs.items() if next(_each_range(value), None))
 
 
-def pretty_parameters(bmark, include_version=True, include_name=True) -> Dict[str, Any]:
-    """Flattened, dash-named parameter dict of a benchmark instance.
+def pretty_parameters(bmark, include_version: bool = True,
+                      include_name: bool = True) -> Dict[str, Any]:
+    """The result columns describing one benchmark instance.
 
     >>> from stencil_benchmarks_amd.benchmark import Benchmark, Parameter
-    >>> class BMark(Benchmark):
-    ...     foo_bar = Parameter('foo param', dtype=int, nargs=1)
-    ...     baz = Parameter('bar param', default=(1, 2))
+    >>> class Probe(Benchmark):
+    ...     grid_size = Parameter("grid", (4, 2))
+    ...     kind = Parameter("kind", "x")
     ...     def run(self):
-    ...         pass
-    >>> pretty_parameters(BMark(foo_bar=42), include_version=False,
-    ...                   include_name=False)
-    {'foo-bar': 42, 'baz-0': 1, 'baz-1': 2}
+    ...         return {}
+    >>> pretty_parameters(Probe(), include_version=False, include_name=False)
+    {'grid-size-0': 4, 'grid-size-1': 2, 'kind': 'x'}
     """
-    columns = {}
+    columns: Dict[str, Any] = {}
     for name, value in bmark.parameters.items():
-        key = name.replace("_", "-")
+        column = name.replace("_", "-")
         if isinstance(value, tuple):
-            for position, item in enumerate(value):
-                columns[f"{key}-{position}"] = item
+            columns.update((f"{column}-{i}", item) for i, item in enumerate(value))
         else:
-            columns[key] = value
+            columns[column] = value
     if include_version:
         columns["sbench-version"] = __version__
     if include_name:
-        columns["benchmark-name"] = type(bmark).__module__ + "." + type(bmark).__name__
+        cls = type(bmark)
+        columns["benchmark-name"] = f"{cls.__module__}.{cls.__name__}"
     return columns
+
+
+class ProgressBar:
+    """Progress of nested loops on one terminal line of standard error.
+
+    ``report(iterable)`` yields the items and advances; nested calls refine
+    the fraction of the enclosing step.
+
+    >>> bar = ProgressBar()
+    >>> bar.fraction
+    0.0
+    """
+
+    WIDTH = 50
+
+    def __init__(self, stream=None):
+        self._stream = stream
+        self._loops: List[List[int]] = []  # [done, total] per nesting level
+
+    @property
+    def fraction(self) -> float:
+        """Completed fraction of the outermost loop, in [0, 1]."""
+        total, scale = 0.0, 1.0
+        for done, count in self._loops:
+            if not count:
+                break
+            total += scale * done / count
+            scale /= count
+        return min(total, 1.0)
+
+    @property
+    def progress(self) -> float:
+        """The completed fraction in percent."""
+        return 100.0 * self.fraction
+
+    def report(self, iterable: Iterable[Any]) -> Iterator[Any]:
+        items = list(iterable)
+        loop = [0, len(items)]
+        self._loops.append(loop)
+        self._draw()
+        try:
+            for item in items:
+                yield item
+                loop[0] += 1
+                self._draw()
+        finally:
+            self._loops.remove(loop)
+
+    def _out(self):
+        return self._stream or sys.stderr
+
+    def _draw(self):
+        filled = int(self.fraction * self.WIDTH)
+        counts = " ".join(f"{done}/{count}" for done, count in self._loops)
+        self._out().write(f"\r[{'#' * filled}{'.' * (self.WIDTH - filled)}] "
+                          f"{self.progress:5.1f}% {counts}  ")
+        self._out().flush()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, exc_type, exc, traceback):
+        # a finished bar is erased; an interrupted one is left on its line
+        self._out().write("\r" + " " * (self.WIDTH + 40) + "\r" if exc_type is None else "\n")
+        self._out().flush()
+        return False

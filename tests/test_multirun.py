@@ -63,13 +63,24 @@ def test_report_groups_and_pivots_a_sweep_csv(tmp_path):
                multirun.Configuration(FakeStencil, name="b", scale=2.0)]
     path = tmp_path / "sweep.csv"
     multirun.run_scaling_benchmark(configs, 2, domain_range=[(4, 4, 2), (8, 8, 2)]).to_csv(path)
-    table, common = sbench_analyze.arrange(sbench_analyze.read_csv(path), groups=("name", "domain-0"),
-                                           unstack=True, select=("bandwidth",))
+    pivot = sbench_analyze.Arrangement(groups=("name", "domain-0"), unstack=True,
+                                       select=("bandwidth",))
+    table = pivot.apply(sbench_analyze.read_csv(path))
     assert table.loc["a", 4] == 32 and table.loc["b", 8] == 256
-    assert common["dtype"] == "float64"
+    assert pivot.common["dtype"] == "float64"
     result = CliRunner().invoke(sbench_analyze.main, ["print", str(path), "-g", "name", "-g",
                                                       "domain-0", "-u", "-s", "bandwidth"])
     assert result.exit_code == 0, result.output
     assert "256" in result.output
-    filtered, _ = sbench_analyze.arrange(sbench_analyze.read_csv(path), query="name == 'a'")
+    filtered = sbench_analyze.Arrangement(query="name == 'a'").apply(
+        sbench_analyze.read_csv(path))
     assert isinstance(filtered, pd.DataFrame) and len(filtered) == 4
+    auto = sbench_analyze.Arrangement(auto_group=True, select=("bandwidth",))
+    grouped = auto.apply(sbench_analyze.read_csv(path))
+    assert grouped.loc[("a", 4, 4)] == 32 and grouped.index.names == ["name", "domain-0", "domain-1"]
+    plot = tmp_path / "plot.png"
+    result = CliRunner().invoke(sbench_analyze.main, ["plot", str(path), "-g", "name", "-g",
+                                                      "domain-0", "-s", "bandwidth", "-r",
+                                                      "peak=300", "-o", str(plot)])
+    assert result.exit_code == 0, result.output
+    assert plot.stat().st_size > 1000
