@@ -135,28 +135,47 @@ def bench_hdiff_headline(steps, warmup):
                 transfer_bytes=devs[0].span + devs[1].span + devs[2].span)
 
 
-def bench_stream(steps):
+# launch shapes of the STREAM sweep (the reference sweeps block size x vector
+# size, stream/cuda_hip.py:41-60, README.md:403-411): threads per block x
+# elements per lane (float64: 1 or 2) x vectors per thread x non-temporal
+# loads and stores x persistent grid (0: one pass, else workgroups per CU)
+STREAM_SWEEP = [(block, vector, unroll, nt, per_cu)
+                for block in (128, 256, 512, 1024)
+                for vector in (1, 2)
+                for unroll in (1, 2, 4)
+                for nt in (0, 3)
+                for per_cu in (0, 8)]
+
+
+def bench_stream(repeats, shapes=None):
+    """STREAM on three 2^29-element float64 arrays over the launch-shape
+    sweep: per kernel the best bandwidth (GB/s) and the shape that gave it,
+    each shape timed `repeats` times after one untimed launch."""
     n = 1 << 29
     bufs = [DeviceBuffer(n * 8) for _ in range(3)]
     for i, b in enumerate(bufs):
         native.call("sb_stream_fill_random", native.SB_F64, ctypes.c_void_p(b.ptr), n, 100 + i)
-    out = {}
     words = {"copy": 2, "scale": 2, "add": 3, "triad": 3}
-    for variant in (0, 3):
+    best, shape_of, table = {}, {}, []
+    for block, vector, unroll, nt, per_cu in shapes or STREAM_SWEEP:
+        opts = native.launch_options(variant=nt, block=(block, vector, per_cu), unroll=unroll)
         for op, code in native.STREAM_OPS.items():
-            opts = native.launch_options(variant=variant, block=(256, 1, 1), unroll=1)
-            best = None
-            for _ in range(max(3, steps)):
+            times = []
+            for _ in range(repeats + 1):
                 seconds = ctypes.c_double(0.0)
                 native.call("sb_stream_kernel", native.SB_F64, code,
                             *[ctypes.c_void_p(b.ptr) for b in bufs], n, 3.0, opts,
                             ctypes.byref(seconds))
-                best = seconds.value if best is None else min(best, seconds.value)
-            key = op if variant == 0 else f"{op}_nt"
-            out[key] = words[op] * n * 8 / best / 1e9
+                times.append(seconds.value)
+            gbs = words[op] * n * 8 / min(times[1:]) / 1e9
+            table.append((op, block, vector, unroll, nt, per_cu, gbs))
+            if gbs > best.get(op, 0.0):
+                best[op] = gbs
+                shape_of[op] = {"block": block, "vector_elements": vector, "unroll": unroll,
+                                "nontemporal": bool(nt), "workgroups_per_cu": per_cu}
     for b in bufs:
         b.free()
-    return out
+    return best, shape_of, table
 
 
 def bench_lap(steps):
@@ -225,48 +244,44 @@ def bench_batched(compute_dtype, storage_dtype, repeats=2):
                           "from_cache": tuned.from_cache, "candidates": len(tuned.trials)}}
 
 
-def bench_plain_steps(steps=10):
-    """The batched workload with the plain FP64 kernels: per subdomain and step
-    one sb_hdiff and one sb_vadv launch; kernel time only (sum of per-launch
-    event times, so launch gaps do not count against the plain path)."""
+def bench_plain_steps(steps=10, storage=np.float64, compute=np.float64):
+    """The batched workload with the plain kernels: per subdomain and step one
+    sb_hdiff and one sb_vadv launch, in the given storage and arithmetic
+    types; kernel time only (sum of per-launch event times, so launch gaps do
+    not count against the plain path)."""
     from stencil_benchmarks_amd import batch
 
+    prec = native.precision_code(storage, compute)
     specs = batch.random_specs(96, 32, 512, 40, 160, seed=0)
-    subs = [batch.Subdomain(s.domain, s.halo, np.float64, seed=s.seed) for s in specs]
+    subs = [batch.Subdomain(s.domain, s.halo, storage, seed=s.seed) for s in specs]
+
+    def one_step(sub, timed):
+        """One hdiff + vadv step of `sub`; their kernel seconds when timed."""
+        f = sub.fields
+        cur, nxt = f[f"u{sub.parity}"], f[f"u{sub.parity ^ 1}"]
+        seconds = [ctypes.c_double(0.0), ctypes.c_double(0.0)]
+        refs = [ctypes.byref(x) if timed else None for x in seconds]
+        native.call("sb_hdiff", prec, ctypes.c_void_p(cur.pointer()),
+                    ctypes.c_void_p(f["coeff"].pointer()), ctypes.c_void_p(nxt.pointer()),
+                    sub.geom, native.launch_options(), refs[0])
+        ptrs = [nxt, f["upos"], f["utens"], f["utensstage"], f["wcon"]]
+        arr = (ctypes.c_void_p * 8)(*([p.pointer() for p in ptrs] + [0, 0, 0]))
+        native.call("sb_vadv", prec, 0, arr, sub.geom, native.launch_options(), refs[1])
+        sub.advance(1)
+        return seconds[0].value + seconds[1].value
+
     # untimed warm-up, as the batched runs get: two steps of every subdomain
     # (the first launches on fresh fields run slower)
     for sub in subs:
-        f = sub.fields
         for _ in range(2):
-            cur, nxt = f[f"u{sub.parity}"], f[f"u{sub.parity ^ 1}"]
-            native.call("sb_hdiff", native.SB_F64, ctypes.c_void_p(cur.pointer()),
-                        ctypes.c_void_p(f["coeff"].pointer()), ctypes.c_void_p(nxt.pointer()),
-                        sub.geom, native.launch_options(), None)
-            ptrs = [nxt, f["upos"], f["utens"], f["utensstage"], f["wcon"]]
-            arr = (ctypes.c_void_p * 8)(*([p.pointer() for p in ptrs] + [0, 0, 0]))
-            native.call("sb_vadv", native.SB_F64, 0, arr, sub.geom, native.launch_options(), None)
-            sub.advance(1)
+            one_step(sub, False)
     native.call("sb_synchronize")
-    sec = ctypes.c_double()
-    total = 0.0
-    for sub in subs:
-        f = sub.fields
-        for _ in range(steps):
-            cur, nxt = f[f"u{sub.parity}"], f[f"u{sub.parity ^ 1}"]
-            native.call("sb_hdiff", native.SB_F64, ctypes.c_void_p(cur.pointer()),
-                        ctypes.c_void_p(f["coeff"].pointer()), ctypes.c_void_p(nxt.pointer()),
-                        sub.geom, native.launch_options(), ctypes.byref(sec))
-            total += sec.value
-            ptrs = [nxt, f["upos"], f["utens"], f["utensstage"], f["wcon"]]
-            arr = (ctypes.c_void_p * 8)(*([p.pointer() for p in ptrs] + [0, 0, 0]))
-            native.call("sb_vadv", native.SB_F64, 0, arr, sub.geom, native.launch_options(),
-                        ctypes.byref(sec))
-            total += sec.value
-            sub.advance(1)
+    total = sum(one_step(sub, True) for sub in subs for _ in range(steps))
     nbytes = steps * sum(s.compulsory_bytes_per_step() for s in subs)
     for s in subs:
         s.free()
-    return {"seconds": total, "gbs": nbytes / total / 1e9}
+    return {"seconds": total, "gbs": nbytes / total / 1e9,
+            "precision": f"{np.dtype(storage).name} storage, {np.dtype(compute).name} arithmetic"}
 
 
 def bench_serving(count=160, load=0.8):
@@ -294,6 +309,9 @@ def bench_serving(count=160, load=0.8):
     served = server().run(stream, realtime=True)
     plain_s, plain_bytes, plain_points = serve.run_plain(
         serve.poisson_requests(rate=0.0, count=count, seed=1), dtype="float64")
+    plain32_s, plain32_bytes, _ = serve.run_plain(
+        serve.poisson_requests(rate=0.0, count=count, seed=1), dtype="float32",
+        compute="float64")
     # open loop (Poisson arrivals at `load` x capacity): latencies; its
     # throughput is the arrival rate, so the speedup over plain FP64 is the
     # closed-loop (capacity) ratio on the same request mix
@@ -305,7 +323,12 @@ def bench_serving(count=160, load=0.8):
         "capacity_gbs": closed.bytes_moved / closed.wall_s / 1e9,
         "plain_f64_requests_per_s": count / plain_s,
         "plain_f64_gbs": plain_bytes / plain_s / 1e9,
+        "plain_f32s_f64c_requests_per_s": count / plain32_s,
+        "plain_f32s_f64c_gbs": plain32_bytes / plain32_s / 1e9,
+        # continuous batching and float32 storage together (north-star ratio)
         "speedup_vs_plain_f64": capacity / (count / plain_s),
+        # continuous batching alone: the same precision served one request at a time
+        "speedup_vs_plain_same_precision": capacity / (count / plain32_s),
         "autotuned": {"hdiff": tuned.hdiff, "vadv": tuned.vadv, "from_cache": tuned.from_cache},
     })
     return out
@@ -321,12 +344,29 @@ def bench_host_stream(threads, array_size=1 << 27):
     return {r["name"]: r["bandwidth"] / 1e3 for r in rows}
 
 
+def host_threads():
+    """Host threads for the CPU legs: OMP_NUM_THREADS when set, else every CPU
+    this process may run on."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+
+
+def _time_cpu(call, budget_s, max_runs=5000):
+    call()  # warm-up (first touch, thread spin-up)
+    times, spent = [], 0.0
+    while spent < budget_s and len(times) < max_runs:
+        t = call()
+        times.append(t)
+        spent += t
+    return float(np.median(times)), len(times), spent
+
+
 def cpu_baseline(headline, budget_s=12.0):
-    """OpenMP port of the reference CPU hdiff on the headline fields."""
+    """OpenMP port of the reference CPU hdiff (Classic 8x8x1) on the headline
+    fields: the `cpu_baseline` figure."""
     from oracle import oracle
 
     lib = oracle.cpu_lib()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     domain, halo = headline["domain"], headline["halo"]
     inp, coeff, out = headline["host"]
     strides = [s // 8 for s in inp.strides]
@@ -335,26 +375,65 @@ def cpu_baseline(headline, budget_s=12.0):
         return ctypes.c_void_p(a.ctypes.data + sum(s * h for s, h in zip(a.strides, halo)))
 
     args = [origin(inp), origin(coeff), origin(out)] + [ctypes.c_ssize_t(v) for v in (*domain, *strides)]
-    lib.cpu_hdiff_f64(*args, ctypes.c_int(threads))  # warm-up (first touch, thread spin-up)
-    times = []
-    spent = 0.0
-    while spent < budget_s and len(times) < 5000:
-        t = lib.cpu_hdiff_f64(*args, ctypes.c_int(threads))
-        times.append(t)
-        spent += t
+    best, runs, spent = _time_cpu(lambda: lib.cpu_hdiff_f64(*args, ctypes.c_int(threads)), budget_s)
     inner = tuple(slice(h, h + d) for d, h in zip(domain, halo))
     diff = float(np.abs(out[inner] - headline["gpu_out"][inner]).max())
-    best = float(np.median(times))
     return {
         "value": headline["bytes"] / best / 1e9,
         "unit": "GB/s",
         "cores": threads,
         "kind": "port",
         "sample": f"hdiff 512x512x160 float64 full domain, OpenMP Classic 8x8x1 port, "
-                  f"{len(times)} runs ({spent:.1f} s), median run",
+                  f"{runs} runs ({spent:.1f} s), median run",
         "seconds_per_pass": best,
         "max_abs_diff_vs_gpu": diff,
     }
+
+
+def cpu_vadv_baseline(threads, budget_s=6.0):
+    """OpenMP port of the reference CPU vadv (KMiddle, blocks of 8 columns)
+    on a 512 x 512 x 160 float64 domain: GB/s of compulsory bytes."""
+    from oracle import oracle
+
+    lib = oracle.cpu_lib()
+    domain, halo = (512, 512, 160), (3, 3, 3)
+    fields = make_host_fields(domain, halo, np.float64, 5, seed=4)
+    strides = [s // 8 for s in fields[0].strides]
+    ptrs = [ctypes.c_void_p(a.ctypes.data + sum(s * h for s, h in zip(a.strides, halo)))
+            for a in fields]
+    args = ptrs + [ctypes.c_ssize_t(v) for v in (*domain, *strides)]
+    best, runs, spent = _time_cpu(lambda: lib.cpu_vadv_f64(*args, ctypes.c_int(threads)),
+                                  budget_s)
+    return {"gbs": vadv_bytes(domain, 8) / best / 1e9, "seconds_per_pass": best,
+            "cores": threads, "kind": "port",
+            "sample": f"vadv 512x512x160 float64, OpenMP KMiddle port, {runs} runs "
+                      f"({spent:.1f} s), median run"}
+
+
+def numpy_config1(budget_s=5.0):
+    """The reference's numpy OnTheFly hdiff at 128 x 128 x 80 (halo 3,
+    float64, layout (0, 1, 2), inp and coeff U[0,1) from seed 0, out zero),
+    timed by perf_counter around the numpy kernel like the reference's
+    tools/timing.py; the reference formula's bytes over the median run."""
+    from stencil_benchmarks_amd.benchmarks_collection.stencils.numpy import horizontal_diffusion
+
+    bench = horizontal_diffusion.OnTheFly(domain=(128, 128, 80), halo=(3, 3, 3),
+                                          dtype="float64", layout=(0, 1, 2), seed=0,
+                                          verify=False)
+    first = bench.run()  # verified once below, timed from the second run on
+    times, spent = [], 0.0
+    while spent < budget_s and len(times) < 200:
+        result = bench.run()
+        times.append(result["time"])
+        spent += result["time"]
+    checked = horizontal_diffusion.OnTheFly(domain=(128, 128, 80), halo=(3, 3, 3),
+                                            dtype="float64", layout=(0, 1, 2), seed=0,
+                                            verify=True)
+    checked.run()
+    best = float(np.median(times))
+    return {"seconds": best, "reference_formula_gbs": bench.data_size / best / 1e9,
+            "runs": len(times), "first_run_s": first["time"], "verified": True,
+            "threads": 1, "note": "numpy elementwise kernels run on one thread"}
 
 
 def profiled_traffic(key):
@@ -414,9 +493,10 @@ def main():
             (2 * points + (nx + 4) * (ny + 4) * (nz + 4)) * 8 / mean_s / 1e9,
         "hdiff_512x512x160_f64_gridpoints_per_s": points / mean_s}
     if not args.quick:
-        log("STREAM 3 x 2^29 float64")
-        extras["stream_gbs"] = bench_stream(args.steps // 4 + 3)
-        log(f"STREAM {extras['stream_gbs']}")
+        log(f"STREAM 3 x 2^29 float64, {len(STREAM_SWEEP)} launch shapes")
+        extras["stream_gbs"], extras["stream_best_shapes"], _ = bench_stream(3)
+        extras["stream_shapes_swept"] = len(STREAM_SWEEP)
+        log(f"STREAM best {extras['stream_gbs']}")
         log("laplacian 512x512x160 float64")
         extras["laplacian_gbs"], _ = bench_lap(args.steps)
         log("vadv")
@@ -434,7 +514,7 @@ def main():
         extras["hdiff_512x512x160_f32s_f64c_gbs"], _ = bench_hdiff_f32(args.steps)
         extras["hdiff_512x512x160_f64_warm_gbs"] = head["bytes"] / float(head["warm_times"].mean()) / 1e9
 
-        triad = max(extras["stream_gbs"]["triad"], extras["stream_gbs"]["triad_nt"])
+        triad = extras["stream_gbs"]["triad"]
         extras["hdiff_frac_of_triad"] = achieved / triad
         extras["vadv_512x512x160_f64_frac_of_triad"] = extras["vadv_512x512x160_f64_gbs"] / triad
         extras["vadv_256x256x80_f64_frac_of_triad"] = extras["vadv_256x256x80_f64_gbs"] / triad
@@ -445,10 +525,19 @@ def main():
         extras["batched_f32s_f64c"] = b32
         extras["batched_f32_speedup_vs_f64"] = b64["seconds"] / b32["seconds"]
         plain = bench_plain_steps()
+        plain32 = bench_plain_steps(storage=np.float32, compute=np.float64)
         extras["plain_f64_same_workload"] = plain
+        extras["plain_f32s_f64c_same_workload"] = plain32
+        # batching alone: batched vs plain kernels in the same precision
+        extras["batched_speedup_vs_plain_same_precision"] = {
+            "f64": plain["seconds"] / b64["seconds"],
+            "f32s_f64c": plain32["seconds"] / b32["seconds"]}
+        # batching and float32 storage together (the north-star ratio): the
+        # float32-storage batch against the plain FP64 kernels
         extras["batched_f32_speedup_vs_plain_f64"] = plain["seconds"] / b32["seconds"]
         log(f"batched f64 {b64['gbs']:.0f} GB/s, f32 {b32['gbs']:.0f} GB/s, plain f64 "
-            f"{plain['gbs']:.0f} GB/s: f32 batched {extras['batched_f32_speedup_vs_plain_f64']:.2f}x")
+            f"{plain['gbs']:.0f} GB/s, plain f32 {plain32['gbs']:.0f} GB/s: f32 batched "
+            f"{extras['batched_f32_speedup_vs_plain_f64']:.2f}x plain f64")
         log("serving: Poisson stream at 80% of capacity")
         extras["serving"] = bench_serving()
         log(f"serving p50 {extras['serving']['p50_latency_ms']:.2f} ms "
@@ -459,6 +548,10 @@ def main():
     if not args.no_cpu:
         log("CPU baseline (OpenMP port) on the headline fields")
         baseline = cpu_baseline(head)
+        log("CPU baseline: vadv KMiddle port")
+        baseline["vadv_512x512x160_f64"] = cpu_vadv_baseline(baseline["cores"])
+        log("CPU baseline: numpy OnTheFly hdiff 128x128x80 (the reference's CPU case)")
+        baseline["numpy_hdiff_128x128x80_f64"] = numpy_config1()
         log("host STREAM (the CPU baseline's memory ceiling)")
         baseline["host_stream_gbs"] = bench_host_stream(baseline["cores"])
         baseline["frac_of_host_triad"] = baseline["value"] / baseline["host_stream_gbs"]["triad"]
@@ -517,7 +610,8 @@ def run_single(args):
     elif args.only == "laplacian":
         out["laplacian_gbs"], out["mean_s"] = bench_lap(args.steps)
     elif args.only == "stream":
-        out["stream_gbs"] = bench_stream(args.steps)
+        out["stream_gbs"], out["stream_best_shapes"], table = bench_stream(3)
+        out["stream_sweep"] = table
     elif args.only == "batched":
         out["batched_f64"] = bench_batched(np.float64, np.float64)
         out["batched_f32s_f64c"] = bench_batched(np.float64, np.float32)

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-#define SB_ABI_VERSION 1
+#define SB_ABI_VERSION 2
 
 enum sb_precision { SB_F64 = 0, SB_F32 = 1, SB_F32_F64 = 2 };
 
@@ -54,17 +54,25 @@ typedef struct sb_geom {
     int64_t hi, hj, hk;
 } sb_geom;
 
+/* Timers of a timed launch (the `timers` choices 'gpu' and 'hip-ext' of
+ * stencils/cuda_hip/mixin.py:57; 'wall' is a host clock around the call).
+ * SB_TIMER_EVENTS records events on the stream before and after the
+ * kernels; SB_TIMER_HIP_EXT has the kernels record them themselves through
+ * hipExtLaunchKernelGGL (templates/base.j2:160-161). */
+#define SB_TIMER_EVENTS 0
+#define SB_TIMER_HIP_EXT 1
+
 /* Launch options; zero means "library default / autotuned".  Replaces the
- * block_size / threads_per_block / unroll_factor / dry_runs parameters
- * (stencils/cuda_hip/basic.py:43-48, horizontal_diffusion.py:40-47,
- * vertical_advection.py:39-57, mixin.py:56). */
+ * block_size / threads_per_block / unroll_factor / dry_runs / timers
+ * parameters (stencils/cuda_hip/basic.py:43-48, horizontal_diffusion.py:40-47,
+ * vertical_advection.py:39-57, mixin.py:56-57). */
 typedef struct sb_launch {
     int32_t variant;   /* kernel variant id, see the per-function notes */
     int32_t bx, by, bz;
     int32_t unroll;
     int32_t dry_runs;
     int32_t flush;     /* 1: evict L2 and Infinity Cache before the timed launch */
-    int32_t reserved;
+    int32_t timer;     /* SB_TIMER_EVENTS (0) or SB_TIMER_HIP_EXT */
 } sb_launch;
 
 typedef struct sb_device_info {
@@ -187,8 +195,13 @@ int sb_vadv(int prec, int all_components, void *const *fields,
  * Replaces the generated STREAM program (benchmarks_collection/stream/cuda_hip.j2:132-288).
  * prec is SB_F64 or SB_F32.  sb_stream_kernel runs one of copy (c = a),
  * scale (b = s c), add (c = a + b), triad (a = b + s c) over n elements.
- * launch->bx = threads per block, ->unroll = vectors per thread,
- * ->variant bit 0 = non-temporal loads, bit 1 = non-temporal stores. */
+ * launch->bx = threads per block (0: 256), ->by = elements per vector
+ * (0: 16 bytes; 1 or 2 for SB_F64, 1, 2 or 4 for SB_F32), ->bz = workgroups
+ * per CU of a persistent grid (0: a grid covering the arrays once),
+ * ->unroll = vectors per thread (0: 4), ->variant bit 0 = non-temporal
+ * loads, bit 1 = non-temporal stores, bits 2-3 = grid axis 0 x, 1 y, 2 z
+ * (block_size / vector_size / unroll_factor / streaming_* / axis of
+ * stream/cuda_hip.py:41-60). */
 int sb_stream_fill(int prec, void *x, size_t n, double value);
 /* x[i] = uniform [0,1) from a counter-based hash of (seed, i); returns after
  * the fill completed. */

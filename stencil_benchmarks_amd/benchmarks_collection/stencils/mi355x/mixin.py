@@ -8,8 +8,10 @@ kernels are prebuilt in libsbmi355x.so and take the geometry at run time.
   field of every data set (same strides as the host field, interior origin
   on a 256-byte boundary) and uploads it once; fields stay resident in HBM.
 * ``run_stencil`` launches the kernel through the C ABI, timed with HIP
-  events on the library stream (``timers='gpu'``, the reference default) or
-  with a host clock around launch and synchronisation (``timers='wall'``),
+  events on the library stream (``timers='gpu'``, the reference default),
+  with events the kernel launches record themselves (``timers='hip-ext'``,
+  hipExtLaunchKernelGGL) or with a host clock around launch and
+  synchronisation (``timers='wall'``),
   after ``dry_runs`` untimed launches and, with ``flush_caches``, after
   streaming 1 GiB through L2 and the Infinity Cache; then it downloads the
   fields the kernel wrote so that ``verify_stencil`` sees them.
@@ -34,7 +36,7 @@ from ....device import DeviceField, geometry
 class StencilMixin(Benchmark):
     device = Parameter("GPU device index", 0)
     dry_runs = Parameter("kernel dry-runs before the measurement", 0)
-    timers = Parameter("timer type", default="gpu", choices=["gpu", "wall"])
+    timers = Parameter("timer type", default="gpu", choices=["gpu", "wall", "hip-ext"])
     flush_caches = Parameter(
         "evict L2 and Infinity Cache before each timed launch (cold-cache timing)", True
     )
@@ -84,14 +86,15 @@ class StencilMixin(Benchmark):
         """Call the C-ABI entry point with the device `pointers` (args order)."""
 
     def launch_options(self):
+        timer = native.SB_TIMER_HIP_EXT if self.timers == "hip-ext" else native.SB_TIMER_EVENTS
         return native.launch_options(variant=self.variant, dry_runs=self.dry_runs,
-                                     flush=self.flush_caches)
+                                     flush=self.flush_caches, timer=timer)
 
     def run_stencil(self, data):
         device_set = self._device_data[self.data_set_index]
         pointers = [ctypes.c_void_p(f.pointer()) for f in device_set]
         options = self.launch_options()
-        if self.timers == "gpu":
+        if self.timers in ("gpu", "hip-ext"):
             seconds = ctypes.c_double(0.0)
             self.launch(pointers, options, ctypes.byref(seconds))
             elapsed = seconds.value

@@ -11,8 +11,15 @@ iteration 0 (stream/cuda_hip.j2:244-276).
 form of stream/cuda_hip.j2:329-345 on the device; ``init='random'`` fills
 a, b, c with U[0,1) from a counter hash of (seed, index) and checks that
 every element follows the recurrence to within the reference's per-type
-epsilon.  ``vector_size`` is fixed by the kernels at 16 bytes per lane per
-load; ``unroll_factor`` is the number of such vectors per thread.
+epsilon.
+
+Launch shape, as the reference's options: ``block_size`` threads per
+workgroup, ``vector_size`` elements per lane and load (0: 16 bytes, i.e. 2
+float64 or 4 float32; 1 or 2 for float64, 1, 2 or 4 for float32),
+``unroll_factor`` vectors per thread, ``axis`` the grid dimension the
+threads and workgroups are laid along, non-temporal loads and stores, and
+``workgroups_per_cu``: 0 launches one pass over the arrays, k > 0 a
+persistent grid of k workgroups per CU that strides over them.
 """
 
 import ctypes
@@ -41,9 +48,12 @@ def closed_form(ntimes, a=1.0, b=2.0, c=0.0, scalar=3.0, dtype=np.float64):
 class Native(Benchmark):
     array_size = Parameter("number of elements in arrays", 10000000)
     ntimes = Parameter("number of runs", 10)
-    block_size = Parameter("threads per block", 256)
+    block_size = Parameter("threads per block", 1024)
     dtype = Parameter("data type in NumPy format, e.g. float32 or float64", "float64")
-    unroll_factor = Parameter("16-byte vectors per thread", 4)
+    axis = Parameter("compute grid dimension to use", "x", choices=["x", "y", "z"])
+    vector_size = Parameter("vector size in elements (0: 16 bytes)", 0)
+    unroll_factor = Parameter("loop unroll factor (vectors per thread)", 1)
+    workgroups_per_cu = Parameter("persistent grid: workgroups per CU (0: one pass)", 0)
     streaming_stores = Parameter("use non-temporal store instructions", False)
     streaming_loads = Parameter("use non-temporal load instructions", False)
     init = Parameter("initial values", "constant", choices=["constant", "random"])
@@ -61,7 +71,13 @@ class Native(Benchmark):
             raise ParameterError("block size must be a multiple of 64 in [64, 1024]")
         if self.unroll_factor not in (1, 2, 4, 8):
             raise ParameterError("unroll factor must be one of 1, 2, 4, 8")
-        vector = 16 // np.dtype(self.dtype).itemsize
+        wide = 16 // np.dtype(self.dtype).itemsize
+        vector = self.vector_size or wide
+        if vector not in (1, 2, wide):
+            raise ParameterError(f"vector size {self.vector_size} not supported for "
+                                 f"{self.dtype} (1, 2 or {wide} elements)")
+        if not 0 <= self.workgroups_per_cu <= 64:
+            raise ParameterError("workgroups per CU must be in 0..64")
         per_block = self.block_size * vector * self.unroll_factor
         if self.array_size % per_block:
             warnings.warn("adapting array size to match block and vector sizes")
@@ -93,8 +109,10 @@ class Native(Benchmark):
 
     def options(self):
         variant = (1 if self.streaming_loads else 0) | (2 if self.streaming_stores else 0)
+        variant |= "xyz".index(self.axis) << 2
         return native.launch_options(
-            variant=variant, block=(self.block_size, 1, 1), unroll=self.unroll_factor
+            variant=variant, block=(self.block_size, self.vector_size, self.workgroups_per_cu),
+            unroll=self.unroll_factor,
         )
 
     def run(self):

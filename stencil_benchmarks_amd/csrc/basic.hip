@@ -374,7 +374,7 @@ int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis
         auto launch_j = [&](auto v_tag) {
             constexpr int VV = decltype(v_tag)::value;
             auto go = [&](auto kernel) {
-                hipLaunchKernelGGL(kernel, grid, dim3(threads), 0, ctx->stream, inp, out, a);
+                enqueue_kernel(kernel, grid, dim3(threads), 0, ctx->stream, inp, out, a);
             };
             if (JB == 8) go(march_kernel<T, C, OP, VV, 8>);
             else if (JB == 16) go(march_kernel<T, C, OP, VV, 16>);
@@ -394,7 +394,7 @@ int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis
     SB_REQUIRE(R == 4 || R == 8 || R == 16, "rows per workgroup %d not in {4, 8, 16}", R);
     const dim3 grid(unsigned((rows + R - 1) / R), unsigned(iblocks));
     auto launch_v = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, grid, dim3(BLOCK), 0, ctx->stream, inp, out, a);
+        enqueue_kernel(kernel, grid, dim3(BLOCK), 0, ctx->stream, inp, out, a);
     };
     auto launch_r = [&](auto v_tag) {
         constexpr int VV = decltype(v_tag)::value;

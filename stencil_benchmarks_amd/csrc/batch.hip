@@ -46,6 +46,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
 #include <vector>
@@ -756,9 +757,14 @@ int sb_batch_step(int prec, const sb_subdomain *subs, int count, const sb_launch
     if (validate(prec, subs, count)) return 2;
     Context *ctx = context();
     if (!ctx) return 1;
-    // one reusable single-step plan per device and thread
-    thread_local Plan *scratch_plan[64] = {};
-    Plan *&p = scratch_plan[ctx->device & 63];
+    // one reusable single-step plan per device, shared by the calling
+    // threads (serialised by a lock; its tables are rebuilt per call and
+    // uploaded in stream order behind the previous call's kernels), held
+    // until the process ends like the device context
+    static std::mutex step_mutex;
+    static Plan *step_plans[64] = {};
+    std::lock_guard<std::mutex> lock(step_mutex);
+    Plan *&p = step_plans[ctx->device & 63];
     if (!p) p = new Plan();
     p->prec = prec;
     p->count = count;

@@ -99,13 +99,13 @@ hipError_t launch_lds(const T *inp, const T *coeff, T *out, HdiffArgs a, Context
     const uint64_t blocks =
         uint64_t(a.tiles_i) * uint64_t(a.tiles_j) * uint64_t((a.nz + KC - 1) / KC);
     switch (flags & 3) {
-        case 0: hipLaunchKernelGGL((hdiff_lds_kernel<T, C, TY, RPT, KC, 0>), dim3(unsigned(blocks)),
+        case 0: enqueue_kernel((hdiff_lds_kernel<T, C, TY, RPT, KC, 0>), dim3(unsigned(blocks)),
                                    dim3(64, TY / RPT), 0, ctx->stream, inp, coeff, out, a); break;
-        case 1: hipLaunchKernelGGL((hdiff_lds_kernel<T, C, TY, RPT, KC, 1>), dim3(unsigned(blocks)),
+        case 1: enqueue_kernel((hdiff_lds_kernel<T, C, TY, RPT, KC, 1>), dim3(unsigned(blocks)),
                                    dim3(64, TY / RPT), 0, ctx->stream, inp, coeff, out, a); break;
-        case 2: hipLaunchKernelGGL((hdiff_lds_kernel<T, C, TY, RPT, KC, 2>), dim3(unsigned(blocks)),
+        case 2: enqueue_kernel((hdiff_lds_kernel<T, C, TY, RPT, KC, 2>), dim3(unsigned(blocks)),
                                    dim3(64, TY / RPT), 0, ctx->stream, inp, coeff, out, a); break;
-        default: hipLaunchKernelGGL((hdiff_lds_kernel<T, C, TY, RPT, KC, 3>), dim3(unsigned(blocks)),
+        default: enqueue_kernel((hdiff_lds_kernel<T, C, TY, RPT, KC, 3>), dim3(unsigned(blocks)),
                                     dim3(64, TY / RPT), 0, ctx->stream, inp, coeff, out, a); break;
     }
     return hipGetLastError();
@@ -170,7 +170,7 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
         SB_REQUIRE(g->nz <= 65535, "on-the-fly hdiff supports nz <= 65535");
         const dim3 grid(unsigned((g->nx + 63) / 64), unsigned((g->ny + 3) / 4), unsigned(g->nz));
         return timed_launch(l, seconds, [&]() -> hipError_t {
-            hipLaunchKernelGGL((hdiff_otf_kernel<T, C>), grid, dim3(64, 4), 0, ctx->stream, inp,
+            enqueue_kernel((hdiff_otf_kernel<T, C>), grid, dim3(64, 4), 0, ctx->stream, inp,
                                coeff, out, a);
             return hipGetLastError();
         });

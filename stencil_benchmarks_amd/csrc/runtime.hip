@@ -7,6 +7,8 @@
 
 namespace sb {
 
+thread_local ExtEvents *t_ext_events = nullptr;
+
 namespace {
 thread_local std::string g_error;
 std::mutex g_ctx_mutex;

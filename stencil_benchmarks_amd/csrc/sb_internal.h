@@ -2,6 +2,7 @@
 // state, per-device context, launch timing and small device utilities.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
@@ -48,6 +49,7 @@ struct Context {
     size_t lds_per_block = 0;
 };
 
+
 // Context of the current device; initialises it on first use.  Returns
 // nullptr (with the error set) on failure.
 Context *context();
@@ -55,8 +57,35 @@ Context *context();
 // Evict L2 and the Infinity Cache by streaming a large scratch buffer.
 int flush_caches(Context *ctx);
 
-// Run `launch` (a callable returning hipError_t after enqueueing one
-// launch on ctx->stream) under the timing contract of the C ABI.
+// Events that the kernel launches of this thread record themselves (the
+// SB_TIMER_HIP_EXT timer): the first launch records `start` as it begins,
+// every launch records `stop` as it ends, so the pair brackets the kernels
+// with no stream operation in between.
+struct ExtEvents {
+    hipEvent_t start, stop;
+    int launches;
+};
+extern thread_local ExtEvents *t_ext_events;
+
+// Enqueue `kernel` on `stream`; under the hip-ext timer through
+// hipExtLaunchKernelGGL with the thread's events.
+template <typename F, typename... Args>
+inline void enqueue_kernel(F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t stream,
+                          Args... args) {
+    if (ExtEvents *ev = t_ext_events) {
+        hipExtLaunchKernelGGL(kernel, grid, block, uint32_t(lds), stream,
+                              ev->launches ? nullptr : ev->start, ev->stop, 0u, args...);
+        ++ev->launches;
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, lds, stream, args...);
+    }
+}
+
+// Run `launch` (a callable returning hipError_t after enqueueing the
+// kernels of one launch on ctx->stream, through enqueue_kernel) under the
+// timing contract of the C ABI: l->timer SB_TIMER_EVENTS records events on
+// the stream around the launch, SB_TIMER_HIP_EXT lets the kernels record
+// them (reference timers 'gpu' and 'hip-ext').
 template <typename Launch>
 int timed_launch(const sb_launch *l, double *seconds, Launch &&launch) {
     Context *ctx = context();
@@ -66,13 +95,25 @@ int timed_launch(const sb_launch *l, double *seconds, Launch &&launch) {
         SB_HIP(launch());
         return 0;
     }
+    const int timer = l ? l->timer : SB_TIMER_EVENTS;
+    SB_REQUIRE(timer == SB_TIMER_EVENTS || timer == SB_TIMER_HIP_EXT, "unknown timer %d",
+               timer);
     for (int d = 0; d < dry; ++d) SB_HIP(launch());
     if (l && l->flush) {
         if (flush_caches(ctx)) return 1;
     }
-    SB_HIP(hipEventRecord(ctx->start, ctx->stream));
-    SB_HIP(launch());
-    SB_HIP(hipEventRecord(ctx->stop, ctx->stream));
+    if (timer == SB_TIMER_HIP_EXT) {
+        ExtEvents ev{ctx->start, ctx->stop, 0};
+        t_ext_events = &ev;
+        const hipError_t e = launch();
+        t_ext_events = nullptr;
+        SB_HIP(e);
+        SB_REQUIRE(ev.launches > 0, "this entry point does not support the hip-ext timer");
+    } else {
+        SB_HIP(hipEventRecord(ctx->start, ctx->stream));
+        SB_HIP(launch());
+        SB_HIP(hipEventRecord(ctx->stop, ctx->stream));
+    }
     SB_HIP(hipEventSynchronize(ctx->stop));
     float ms = 0.f;
     SB_HIP(hipEventElapsedTime(&ms, ctx->start, ctx->stop));

@@ -1,16 +1,23 @@
 // STREAM copy / scale / add / triad on HBM-resident arrays.
 //
-// Each thread moves U vectors of V elements (16 bytes per vector, so one
-// global_load/store_dwordx4 each); the U vectors of a thread are one block
-// width apart so every load instruction of a wave covers 1 KiB contiguously.
-// Optional non-temporal hints keep the streamed lines from displacing L2.
+// Each thread moves U vectors of V elements per pass (V elements of 4 or 8
+// bytes: one global_load/store_dword{,x2,x4} per vector); the U vectors of a
+// thread are one block width apart so every load instruction of a wave covers
+// 64 V element-widths contiguously.  Passes stride over the arrays by the
+// whole grid, so the same kernel runs a grid that covers the arrays once or
+// a persistent grid of a few workgroups per CU.  Optional non-temporal hints
+// keep the streamed lines from displacing L2.  The thread and block index
+// come from one grid axis (x, y or z), as the reference's `axis` option
+// chooses.
+#include <type_traits>
+
 #include "sb_internal.h"
 
 namespace sb {
 namespace {
 
 template <typename T, int V>
-using vec_t = T __attribute__((ext_vector_type(V)));
+using vec_t = std::conditional_t<V == 1, T, T __attribute__((ext_vector_type(V)))>;
 
 template <bool NT, typename VT>
 __device__ __forceinline__ VT load(const VT *p) {
@@ -28,45 +35,60 @@ __device__ __forceinline__ void store(VT *p, VT v) {
         *p = v;
 }
 
+struct GridAxis {
+    uint32_t thread, block_size;
+    size_t block, blocks;
+};
+
+__device__ __forceinline__ GridAxis grid_axis(int axis) {
+    if (axis == 1) return {threadIdx.y, blockDim.y, blockIdx.y, gridDim.y};
+    if (axis == 2) return {threadIdx.z, blockDim.z, blockIdx.z, gridDim.z};
+    return {threadIdx.x, blockDim.x, blockIdx.x, gridDim.x};
+}
+
 // c = a; b = s c; c = a + b; a = b + s c  (stream/cuda_hip.j2:147-173)
 template <typename T, int OP, int V, int U, bool NTL, bool NTS>
 __global__ __launch_bounds__(1024) void stream_kernel(T *__restrict__ a, T *__restrict__ b,
-                                                      T *__restrict__ c, T s, size_t nvec) {
+                                                      T *__restrict__ c, T s, size_t nvec,
+                                                      int axis) {
     using VT = vec_t<T, V>;
     VT *A = reinterpret_cast<VT *>(a);
     VT *B = reinterpret_cast<VT *>(b);
     VT *Cv = reinterpret_cast<VT *>(c);
-    const size_t base = size_t(blockIdx.x) * (size_t(U) * blockDim.x) + threadIdx.x;
-    VT x[U], y[U];
+    const GridAxis g = grid_axis(axis);
+    const size_t pass = size_t(U) * g.block_size;
+    for (size_t base = g.block * pass + g.thread; base < nvec; base += g.blocks * pass) {
+        VT x[U], y[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const size_t idx = base + size_t(u) * blockDim.x;
-        if (idx < nvec) {
-            if constexpr (OP == SB_STREAM_COPY) {
-                x[u] = load<NTL>(A + idx);
-            } else if constexpr (OP == SB_STREAM_SCALE) {
-                x[u] = load<NTL>(Cv + idx);
-            } else if constexpr (OP == SB_STREAM_ADD) {
-                x[u] = load<NTL>(A + idx);
-                y[u] = load<NTL>(B + idx);
-            } else {
-                x[u] = load<NTL>(B + idx);
-                y[u] = load<NTL>(Cv + idx);
+        for (int u = 0; u < U; ++u) {
+            const size_t idx = base + size_t(u) * g.block_size;
+            if (idx < nvec) {
+                if constexpr (OP == SB_STREAM_COPY) {
+                    x[u] = load<NTL>(A + idx);
+                } else if constexpr (OP == SB_STREAM_SCALE) {
+                    x[u] = load<NTL>(Cv + idx);
+                } else if constexpr (OP == SB_STREAM_ADD) {
+                    x[u] = load<NTL>(A + idx);
+                    y[u] = load<NTL>(B + idx);
+                } else {
+                    x[u] = load<NTL>(B + idx);
+                    y[u] = load<NTL>(Cv + idx);
+                }
             }
         }
-    }
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-        const size_t idx = base + size_t(u) * blockDim.x;
-        if (idx < nvec) {
-            if constexpr (OP == SB_STREAM_COPY) {
-                store<NTS>(Cv + idx, x[u]);
-            } else if constexpr (OP == SB_STREAM_SCALE) {
-                store<NTS>(B + idx, s * x[u]);
-            } else if constexpr (OP == SB_STREAM_ADD) {
-                store<NTS>(Cv + idx, x[u] + y[u]);
-            } else {
-                store<NTS>(A + idx, x[u] + s * y[u]);
+        for (int u = 0; u < U; ++u) {
+            const size_t idx = base + size_t(u) * g.block_size;
+            if (idx < nvec) {
+                if constexpr (OP == SB_STREAM_COPY) {
+                    store<NTS>(Cv + idx, x[u]);
+                } else if constexpr (OP == SB_STREAM_SCALE) {
+                    store<NTS>(B + idx, s * x[u]);
+                } else if constexpr (OP == SB_STREAM_ADD) {
+                    store<NTS>(Cv + idx, x[u] + y[u]);
+                } else {
+                    store<NTS>(A + idx, x[u] + s * y[u]);
+                }
             }
         }
     }
@@ -173,72 +195,105 @@ __global__ __launch_bounds__(256) void check_kernel(const T *x, size_t n, double
         atomicMax(result, (unsigned long long)__double_as_longlong(worst));
 }
 
+// Largest grid extent along y and z (HIP limit); larger x extents are fine.
+constexpr size_t kMaxGridYZ = 65535;
+
 template <typename T, int OP, int V, int U>
 int launch_stream(T *a, T *b, T *c, size_t n, double scalar, const sb_launch *l,
-                  double *seconds, int block) {
+                  double *seconds, int block, int per_cu, int axis) {
     const size_t nvec = n / V;
-    const size_t per_block = size_t(U) * block;
-    const size_t blocks = (nvec + per_block - 1) / per_block;
+    const size_t pass = size_t(U) * block;
     const int nt = l ? (l->variant & 3) : 0;
     Context *ctx = context();
     if (!ctx) return 1;
+    size_t blocks = (nvec + pass - 1) / pass;
+    if (per_cu > 0) blocks = std::min(blocks, size_t(per_cu) * size_t(ctx->compute_units));
+    if (axis != 0) blocks = std::min(blocks, kMaxGridYZ);
+    const unsigned gb = unsigned(blocks), tb = unsigned(block);
+    const dim3 grid = axis == 0 ? dim3(gb) : axis == 1 ? dim3(1, gb) : dim3(1, 1, gb);
+    const dim3 threads = axis == 0 ? dim3(tb) : axis == 1 ? dim3(1, tb) : dim3(1, 1, tb);
     const T s = T(scalar);
     return timed_launch(l, seconds, [&]() -> hipError_t {
         if (blocks) {
             switch (nt) {
                 case 0:
-                    hipLaunchKernelGGL((stream_kernel<T, OP, V, U, false, false>), dim3(blocks),
-                                       dim3(block), 0, ctx->stream, a, b, c, s, nvec);
+                    enqueue_kernel((stream_kernel<T, OP, V, U, false, false>), grid, threads, 0,
+                                   ctx->stream, a, b, c, s, nvec, axis);
                     break;
                 case 1:
-                    hipLaunchKernelGGL((stream_kernel<T, OP, V, U, true, false>), dim3(blocks),
-                                       dim3(block), 0, ctx->stream, a, b, c, s, nvec);
+                    enqueue_kernel((stream_kernel<T, OP, V, U, true, false>), grid, threads, 0,
+                                   ctx->stream, a, b, c, s, nvec, axis);
                     break;
                 case 2:
-                    hipLaunchKernelGGL((stream_kernel<T, OP, V, U, false, true>), dim3(blocks),
-                                       dim3(block), 0, ctx->stream, a, b, c, s, nvec);
+                    enqueue_kernel((stream_kernel<T, OP, V, U, false, true>), grid, threads, 0,
+                                   ctx->stream, a, b, c, s, nvec, axis);
                     break;
                 default:
-                    hipLaunchKernelGGL((stream_kernel<T, OP, V, U, true, true>), dim3(blocks),
-                                       dim3(block), 0, ctx->stream, a, b, c, s, nvec);
+                    enqueue_kernel((stream_kernel<T, OP, V, U, true, true>), grid, threads, 0,
+                                   ctx->stream, a, b, c, s, nvec, axis);
                     break;
             }
         }
         if (nvec * V < n)
-            hipLaunchKernelGGL((stream_tail<T, OP>), dim3(1), dim3(V), 0, ctx->stream, a, b, c,
-                               s, nvec * V, n);
+            enqueue_kernel((stream_tail<T, OP>), dim3(1), dim3(V), 0, ctx->stream, a, b, c, s,
+                           nvec * V, n);
         return hipGetLastError();
     });
 }
 
-template <typename T, int OP>
+template <typename T, int OP, int V>
 int dispatch_unroll(T *a, T *b, T *c, size_t n, double scalar, const sb_launch *l,
+                    double *seconds, int block, int per_cu, int axis) {
+    const int unroll = (l && l->unroll) ? l->unroll : 4;
+    switch (unroll) {
+        case 1: return launch_stream<T, OP, V, 1>(a, b, c, n, scalar, l, seconds, block, per_cu, axis);
+        case 2: return launch_stream<T, OP, V, 2>(a, b, c, n, scalar, l, seconds, block, per_cu, axis);
+        case 4: return launch_stream<T, OP, V, 4>(a, b, c, n, scalar, l, seconds, block, per_cu, axis);
+        case 8: return launch_stream<T, OP, V, 8>(a, b, c, n, scalar, l, seconds, block, per_cu, axis);
+        default: set_error("STREAM unroll %d not in {1, 2, 4, 8}", unroll); return 2;
+    }
+}
+
+// launch->by: elements per vector (0: 16 bytes), ->bz: workgroups per CU of
+// a persistent grid (0: one pass over the arrays), ->variant bits 2-3: the
+// grid axis (0 x, 1 y, 2 z).
+template <typename T, int OP>
+int dispatch_vector(T *a, T *b, T *c, size_t n, double scalar, const sb_launch *l,
                     double *seconds) {
-    constexpr int V = 16 / sizeof(T);
     const int block = (l && l->bx) ? l->bx : 256;
     SB_REQUIRE(block >= 64 && block <= 1024 && block % 64 == 0,
                "STREAM block size %d must be a multiple of 64 in [64, 1024]", block);
-    const int unroll = (l && l->unroll) ? l->unroll : 4;
-    switch (unroll) {
-        case 1: return launch_stream<T, OP, V, 1>(a, b, c, n, scalar, l, seconds, block);
-        case 2: return launch_stream<T, OP, V, 2>(a, b, c, n, scalar, l, seconds, block);
-        case 4: return launch_stream<T, OP, V, 4>(a, b, c, n, scalar, l, seconds, block);
-        case 8: return launch_stream<T, OP, V, 8>(a, b, c, n, scalar, l, seconds, block);
-        default: set_error("STREAM unroll %d not in {1, 2, 4, 8}", unroll); return 2;
+    constexpr int kWide = int(16 / sizeof(T));
+    const int vec = (l && l->by) ? l->by : kWide;
+    const int per_cu = l ? l->bz : 0;
+    SB_REQUIRE(per_cu >= 0 && per_cu <= 64, "STREAM workgroups per CU %d not in 0..64", per_cu);
+    const int axis = l ? (l->variant >> 2) & 3 : 0;
+    SB_REQUIRE(axis <= 2, "STREAM grid axis %d not in 0..2", axis);
+    for (T *p : {a, b, c})
+        SB_REQUIRE(reinterpret_cast<uintptr_t>(p) % (size_t(vec) * sizeof(T)) == 0,
+                   "STREAM arrays must be aligned to the vector size");
+    switch (vec) {
+        case 1: return dispatch_unroll<T, OP, 1>(a, b, c, n, scalar, l, seconds, block, per_cu, axis);
+        case 2: return dispatch_unroll<T, OP, 2>(a, b, c, n, scalar, l, seconds, block, per_cu, axis);
+        case 4:
+            if constexpr (sizeof(T) == 4)
+                return dispatch_unroll<T, OP, 4>(a, b, c, n, scalar, l, seconds, block, per_cu, axis);
+            [[fallthrough]];
+        default:
+            set_error("STREAM vector size %d not in {1, %d} elements of %zu bytes", vec, kWide,
+                      sizeof(T));
+            return 2;
     }
 }
 
 template <typename T>
 int dispatch_op(int op, T *a, T *b, T *c, size_t n, double scalar, const sb_launch *l,
                 double *seconds) {
-    for (T *p : {a, b, c})
-        SB_REQUIRE(reinterpret_cast<uintptr_t>(p) % 16 == 0,
-                   "STREAM arrays must be 16-byte aligned");
     switch (op) {
-        case SB_STREAM_COPY: return dispatch_unroll<T, SB_STREAM_COPY>(a, b, c, n, scalar, l, seconds);
-        case SB_STREAM_SCALE: return dispatch_unroll<T, SB_STREAM_SCALE>(a, b, c, n, scalar, l, seconds);
-        case SB_STREAM_ADD: return dispatch_unroll<T, SB_STREAM_ADD>(a, b, c, n, scalar, l, seconds);
-        case SB_STREAM_TRIAD: return dispatch_unroll<T, SB_STREAM_TRIAD>(a, b, c, n, scalar, l, seconds);
+        case SB_STREAM_COPY: return dispatch_vector<T, SB_STREAM_COPY>(a, b, c, n, scalar, l, seconds);
+        case SB_STREAM_SCALE: return dispatch_vector<T, SB_STREAM_SCALE>(a, b, c, n, scalar, l, seconds);
+        case SB_STREAM_ADD: return dispatch_vector<T, SB_STREAM_ADD>(a, b, c, n, scalar, l, seconds);
+        case SB_STREAM_TRIAD: return dispatch_vector<T, SB_STREAM_TRIAD>(a, b, c, n, scalar, l, seconds);
         default: set_error("invalid STREAM kernel %d", op); return 2;
     }
 }

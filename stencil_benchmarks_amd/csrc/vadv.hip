@@ -37,6 +37,8 @@
 // the compute wave's progress counter in LDS with bounded waits (it never
 // blocks the compute wave), and after the forward sweep it does the same
 // for the pos lines of the backward substitution.
+#include <algorithm>
+
 #include "vadv_column.h"
 #include "vadv_ws.h"
 
@@ -262,6 +264,10 @@ __device__ float helper_wave(const Component &cp, const T *wcon, int64_t row_bas
     return acc;
 }
 
+// Variants 1 and 2: one 64-column tile per workgroup at a time; workgroups
+// stride over the tiles by the grid size (one pass where the grid covers
+// them all).  Variant 2 keeps (c', d') in global scratch, one slice of 64
+// columns per workgroup, reused for each of its tiles.
 template <typename T, typename C, bool LDS, int PF, bool NT, bool HELPER>
 __global__ __launch_bounds__(HELPER ? 128 : 64) __attribute__((amdgpu_waves_per_eu(1, LDS ? 2 : 8)))
 void vadv_kernel(Components comps, const T *__restrict__ wcon, CD<C> *__restrict__ gscratch,
@@ -270,13 +276,6 @@ void vadv_kernel(Components comps, const T *__restrict__ wcon, CD<C> *__restrict
     __shared__ int s_progress;
     const int lane = threadIdx.x & 63;
     const int wave = HELPER ? int(threadIdx.x >> 6) : 0;
-    const int64_t tile_i = blockIdx.x % a.tiles_i;
-    const int64_t j = blockIdx.x / a.tiles_i;
-    const int64_t i0 = tile_i * 64;
-    const int64_t i = i0 + lane;
-    const bool active = i < a.nx;
-    const int64_t ic = active ? i : a.nx - 1;  // idle lanes re-read a valid column
-    const int64_t base = ic * a.si + j * a.sj;
     CD<C> *scratch;
     int64_t sstride;
     if constexpr (LDS) {
@@ -286,22 +285,31 @@ void vadv_kernel(Components comps, const T *__restrict__ wcon, CD<C> *__restrict
         scratch = gscratch + int64_t(blockIdx.x) * 64 + lane;
         sstride = a.scratch_cols;
     }
-    for (int c = 0; c < a.ncomp; ++c) {
-        if constexpr (HELPER) {
-            if (threadIdx.x == 0) s_progress = -(1 << 30);
-            __syncthreads();
-            if (wave == 0)
+    for (int64_t tile = blockIdx.x; tile < a.tiles; tile += gridDim.x) {
+        const int64_t tile_i = tile % a.tiles_i;
+        const int64_t j = tile / a.tiles_i;
+        const int64_t i0 = tile_i * 64;
+        const int64_t i = i0 + lane;
+        const bool active = i < a.nx;
+        const int64_t ic = active ? i : a.nx - 1;  // idle lanes re-read a valid column
+        const int64_t base = ic * a.si + j * a.sj;
+        for (int c = 0; c < a.ncomp; ++c) {
+            if constexpr (HELPER) {
+                if (threadIdx.x == 0) s_progress = -(1 << 30);
+                __syncthreads();
+                if (wave == 0)
+                    solve_column<T, C, PF, NT>(comps.c[c], wcon, base, a.sk, int(a.nz), scratch,
+                                               sstride, active, &s_progress);
+                else {
+                    const float sum = helper_wave<T>(comps.c[c], wcon, j * a.sj, i0, a, &s_progress);
+                    // an empty asm that reads the sum keeps the touch loads alive
+                    __asm__ volatile("" : : "v"(sum));
+                }
+                __syncthreads();
+            } else {
                 solve_column<T, C, PF, NT>(comps.c[c], wcon, base, a.sk, int(a.nz), scratch,
-                                           sstride, active, &s_progress);
-            else {
-                const float sum = helper_wave<T>(comps.c[c], wcon, j * a.sj, i0, a, &s_progress);
-                // an empty asm that reads the sum keeps the touch loads alive
-                __asm__ volatile("" : : "v"(sum));
+                                           sstride, active, nullptr);
             }
-            __syncthreads();
-        } else {
-            solve_column<T, C, PF, NT>(comps.c[c], wcon, base, a.sk, int(a.nz), scratch, sstride,
-                                       active, nullptr);
         }
     }
 }
@@ -381,10 +389,10 @@ hipError_t launch_kernel(bool lds, uint64_t blocks, size_t lds_bytes, const Comp
         (void)hipFuncSetAttribute(
             reinterpret_cast<const void *>(&vadv_kernel<T, C, true, PF, NT, HELPER>),
             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
-        hipLaunchKernelGGL((vadv_kernel<T, C, true, PF, NT, HELPER>), dim3(unsigned(blocks)),
+        enqueue_kernel((vadv_kernel<T, C, true, PF, NT, HELPER>), dim3(unsigned(blocks)),
                            dim3(HELPER ? 128 : 64), lds_bytes, ctx->stream, comps, wcon, gscratch, a);
     } else {
-        hipLaunchKernelGGL((vadv_kernel<T, C, false, PF, NT, false>), dim3(unsigned(blocks)),
+        enqueue_kernel((vadv_kernel<T, C, false, PF, NT, false>), dim3(unsigned(blocks)),
                            dim3(64), 0, ctx->stream, comps, wcon, gscratch, a);
     }
     return hipGetLastError();
@@ -414,7 +422,7 @@ hipError_t launch_pipe_k(unsigned grid, size_t lds_bytes, const Components &comp
                          const PipeArgs &pa, Context *ctx) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&vadv_pipe_kernel<T, C, PF, NT>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
-    hipLaunchKernelGGL((vadv_pipe_kernel<T, C, PF, NT>), dim3(grid), dim3(64), lds_bytes,
+    enqueue_kernel((vadv_pipe_kernel<T, C, PF, NT>), dim3(grid), dim3(64), lds_bytes,
                        ctx->stream, comps, wcon, pa);
     return hipGetLastError();
 }
@@ -469,7 +477,7 @@ hipError_t launch_ws_k(int per_cu_limit, uint64_t blocks, size_t lds_bytes, cons
         blocks, uint64_t(ctx->compute_units) * uint64_t(per_cu), ctx->xcds);
     PipeArgs args = pa;
     args.xcds = pg.xcds;
-    hipLaunchKernelGGL((vadv_ws_kernel<T, C, NT, FAST, MB>), dim3(pg.grid), dim3(64 * kWsWaves),
+    enqueue_kernel((vadv_ws_kernel<T, C, NT, FAST, MB>), dim3(pg.grid), dim3(64 * kWsWaves),
                        lds_bytes, ctx->stream, comps, wcon, args);
     return hipGetLastError();
 }
@@ -494,30 +502,32 @@ hipError_t launch_ws(int block, bool nt, bool fast, int per_cu_limit, uint64_t b
                                        ctx);
 }
 
-template <int P>
-int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_launch *l,
-                double *seconds) {
-    using T = typename Prec<P>::T;
-    using C = typename Prec<P>::C;
+// The checked fields and geometry of one vadv call.
+template <typename T>
+struct VadvCall {
+    Components comps{};
+    const T *wcon = nullptr;
+    VadvArgs a{};
+    uint64_t blocks = 0;  // 64-column tiles
+    bool empty = false;   // nothing to compute
+};
+
+template <typename T>
+int prepare_vadv(int all_components, void *const *f, const sb_geom *g, VadvCall<T> &call) {
     SB_REQUIRE(f, "NULL field array");
     const int ncomp = all_components ? 3 : 1;
     for (int q = 0; q < 4 * ncomp + 1; ++q) SB_REQUIRE(f[q], "NULL field pointer (argument %d)", q);
-    Context *ctx = context();
-    if (!ctx) return 1;
     if (g->nx == 0 || g->ny == 0 || g->nz == 0) {
-        if (seconds) *seconds = 0.0;
+        call.empty = true;
         return 0;
     }
     SB_REQUIRE(g->nz >= 2, "vertical advection needs nz >= 2 (got %lld)", (long long)g->nz);
     SB_REQUIRE(g->nz <= (1 << 24), "nz too large");
-
-    Components comps{};
     const int64_t shifts[3] = {g->si, g->sj, 0};
     for (int q = 0; q < ncomp; ++q)
-        comps.c[q] = Component{f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3], shifts[q]};
-    const T *wcon = static_cast<const T *>(f[4 * ncomp]);
-
-    VadvArgs a{};
+        call.comps.c[q] = Component{f[4 * q], f[4 * q + 1], f[4 * q + 2], f[4 * q + 3], shifts[q]};
+    call.wcon = static_cast<const T *>(f[4 * ncomp]);
+    VadvArgs &a = call.a;
     a.nx = g->nx;
     a.ny = g->ny;
     a.nz = g->nz;
@@ -526,8 +536,29 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
     a.sk = g->sk;
     a.tiles_i = int32_t((g->nx + 63) / 64);
     a.ncomp = ncomp;
-    const uint64_t blocks = uint64_t(a.tiles_i) * uint64_t(g->ny);
-    SB_REQUIRE(blocks < (uint64_t(1) << 26), "domain too large for one launch");
+    call.blocks = uint64_t(a.tiles_i) * uint64_t(g->ny);
+    a.tiles = int64_t(call.blocks);
+    SB_REQUIRE(call.blocks < (uint64_t(1) << 26), "domain too large for one launch");
+    return 0;
+}
+
+template <int P>
+int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_launch *l,
+                double *seconds) {
+    using T = typename Prec<P>::T;
+    using C = typename Prec<P>::C;
+    VadvCall<T> call;
+    if (int status = prepare_vadv<T>(all_components, f, g, call)) return status;
+    Context *ctx = context();
+    if (!ctx) return 1;
+    if (call.empty) {
+        if (seconds) *seconds = 0.0;
+        return 0;
+    }
+    const Components &comps = call.comps;
+    const T *wcon = call.wcon;
+    VadvArgs a = call.a;
+    const uint64_t blocks = call.blocks;
     // column groups of the pipelined and warp-specialised solvers (rows packed)
     const int64_t groups = vadv_groups(g->nx, g->ny);
 
@@ -634,6 +665,7 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
 }
 
 }  // namespace
+
 }  // namespace sb
 
 using namespace sb;

@@ -14,6 +14,7 @@ struct VadvArgs {
     int32_t tiles_i;
     int32_t ncomp;
     int64_t scratch_cols;  // columns of the global scratch (variant 2)
+    int64_t tiles;         // 64-column tiles (tiles_i x ny); workgroups stride over them
 };
 
 struct Component {

@@ -34,7 +34,7 @@ class Geom(ctypes.Structure):
 class Launch(ctypes.Structure):
     _fields_ = [
         (n, ctypes.c_int32)
-        for n in ("variant", "bx", "by", "bz", "unroll", "dry_runs", "flush", "reserved")
+        for n in ("variant", "bx", "by", "bz", "unroll", "dry_runs", "flush", "timer")
     ]
 
 
@@ -205,9 +205,14 @@ class Event:
             self.handle = None
 
 
-def launch_options(variant=0, block=(0, 0, 0), unroll=0, dry_runs=0, flush=False) -> Launch:
+SB_TIMER_EVENTS, SB_TIMER_HIP_EXT = 0, 1
+
+
+def launch_options(variant=0, block=(0, 0, 0), unroll=0, dry_runs=0, flush=False,
+                   timer=SB_TIMER_EVENTS) -> Launch:
     bx, by, bz = (tuple(block) + (0, 0, 0))[:3]
-    return Launch(int(variant), int(bx), int(by), int(bz), int(unroll), int(dry_runs), int(bool(flush)), 0)
+    return Launch(int(variant), int(bx), int(by), int(bz), int(unroll), int(dry_runs),
+                  int(bool(flush)), int(timer))
 
 
 def precision_code(dtype, compute=None) -> int:

@@ -147,6 +147,9 @@ class Server:
         measures capacity).
         """
         requests = list(requests)
+        for req in requests:
+            if req.steps < 1:
+                raise ValueError(f"request {req.rid}: steps must be at least 1, got {req.steps}")
         budget = self.budget
         if budget is None:
             free, _ = native.mem_info()
@@ -223,16 +226,17 @@ class Server:
                           point_steps, max_active)
 
 
-def run_plain(requests, dtype="float64", halo=(3, 3, 3)):
+def run_plain(requests, dtype="float64", compute=None, halo=(3, 3, 3)):
     """Serve requests one at a time with the plain (unbatched) kernels.
 
     Each step is one ``sb_hdiff`` and one ``sb_vadv`` launch on the
     request's own fields, which come from a device arena allocated before
-    the clock starts (as the server's); returns (wall seconds, bytes,
-    point-steps).
+    the clock starts (as the server's), in storage type `dtype` with
+    arithmetic type `compute` (default: the storage type); returns (wall
+    seconds, bytes, point-steps).
     """
     dtype = np.dtype(dtype)
-    prec = native.precision_code(dtype)
+    prec = native.precision_code(dtype, compute)
     total_bytes = 0.0
     point_steps = 0.0
     arena = DeviceArena(max(footprint_bytes(r.domain, halo, dtype) for r in requests))

@@ -338,20 +338,28 @@ def test_vadv_tall_column_uses_global_scratch(gpu):
     assert np.array_equal(got[inner(domain, halo)], expected)
 
 
+# (unroll, non-temporal bits, elements per vector (0: 16 bytes), workgroups
+# per CU of a persistent grid (0: one pass), grid axis)
+STREAM_SHAPES = [(1, 0, 0, 0, 0), (4, 3, 0, 0, 0), (8, 1, 0, 0, 0), (2, 0, 1, 0, 0),
+                 (1, 3, 2, 2, 0), (4, 0, 1, 1, 0), (1, 0, 0, 0, 1), (2, 3, 1, 0, 2),
+                 (1, 0, 0, 3, 2)]
+
+
 @pytest.mark.parametrize("dtype", ["float64", "float32"])
 @pytest.mark.parametrize("op", ["copy", "scale", "add", "triad"])
-@pytest.mark.parametrize("unroll,nt", [(1, 0), (4, 3), (8, 1)])
-def test_stream_kernels(gpu, dtype, op, unroll, nt):
+@pytest.mark.parametrize("unroll,nt,vector,per_cu,axis", STREAM_SHAPES)
+def test_stream_kernels(gpu, dtype, op, unroll, nt, vector, per_cu, axis):
     from stencil_benchmarks_amd.device import DeviceBuffer
 
-    n = 100003
+    # large enough for several passes of a persistent grid along y or z
+    n = 100003 if per_cu == 0 else 3_000_017
     code = gpu.SB_F64 if dtype == "float64" else gpu.SB_F32
     rng = np.random.default_rng(5)
     host = [rng.random(n).astype(dtype) for _ in range(3)]
     bufs = [DeviceBuffer(n * host[0].itemsize) for _ in range(3)]
     for b, h in zip(bufs, host):
         gpu.call("sb_memcpy_h2d", ctypes.c_void_p(b.ptr), ctypes.c_void_p(h.ctypes.data), h.nbytes)
-    opts = gpu.launch_options(variant=nt, block=(256, 1, 1), unroll=unroll)
+    opts = gpu.launch_options(variant=nt | axis << 2, block=(256, vector, per_cu), unroll=unroll)
     gpu.call("sb_stream_kernel", code, gpu.STREAM_OPS[op], *[ctypes.c_void_p(b.ptr) for b in bufs],
              n, 3.0, opts, None)
     gpu.call("sb_synchronize")
@@ -366,6 +374,19 @@ def test_stream_kernels(gpu, dtype, op, unroll, nt):
                 "triad": (b + s * c, b, c)}[op]
     for g, e in zip(got, expected):
         assert np.array_equal(g, e)
+
+
+def test_stream_rejects_bad_shapes(gpu):
+    from stencil_benchmarks_amd.benchmark import ExecutionError
+    from stencil_benchmarks_amd.device import DeviceBuffer
+
+    bufs = [DeviceBuffer(4096) for _ in range(3)]
+    ptrs = [ctypes.c_void_p(b.ptr) for b in bufs]
+    for opts in (gpu.launch_options(block=(256, 4, 0)),      # 4 doubles per vector
+                 gpu.launch_options(block=(256, 0, 65)),     # persistent grid too large
+                 gpu.launch_options(variant=3 << 2)):        # no fourth axis
+        with pytest.raises(ExecutionError):
+            gpu.call("sb_stream_kernel", gpu.SB_F64, 0, *ptrs, 512, 3.0, opts, None)
 
 
 def test_fill_random_fields_matches_host_hash(gpu):
