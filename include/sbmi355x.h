@@ -191,6 +191,24 @@ int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
 int sb_vadv(int prec, int all_components, void *const *fields,
             const sb_geom *g, const sb_launch *l, double *seconds);
 
+/* sb_vadv without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as sb_vadv with the global-scratch solver (variant 2, bit for bit
+ * the same arithmetic); its (c', d') scratch is a 64 MiB array of the
+ * library's code object (one per device, held until the process ends)
+ * instead of an allocation, and a persistent grid of
+ * 64 MiB / (64 nz 16 bytes) workgroups (FP64 arithmetic) strides over the
+ * columns.  Rejects nz above 65536 (FP64 arithmetic) or 131072 (float32) and
+ * launch->flush (cache flushing allocates).  launch->variant 0 or 2,
+ * ->unroll prefetch depth 4/8/16, ->bz bit 0 non-temporal loads.  Calls on
+ * one device share the array; the library orders its calls on one stream, so
+ * calls overlap (and race) only if a caller runs them on other streams or
+ * processes at once.  The array keeps the last call's contents; nothing in
+ * it needs to start at zero. */
+int sb_vadv_noalloc(int prec, int all_components, void *const *fields,
+                    const sb_geom *g, const sb_launch *l, double *seconds);
+
 /* ---- STREAM ---------------------------------------------------------------
  * Replaces the generated STREAM program (benchmarks_collection/stream/cuda_hip.j2:132-288).
  * prec is SB_F64 or SB_F32.  sb_stream_kernel runs one of copy (c = a),

@@ -48,6 +48,7 @@ int init_context(Context &c, int device) {
               std::strncmp(p.gcnArchName, "gfx95", 5) == 0)
                  ? 8
                  : 1;
+    vadv_noalloc_setup(c);
     return 0;
 }
 }  // namespace

@@ -47,8 +47,16 @@ struct Context {
     int compute_units = 0;
     int xcds = 1;
     size_t lds_per_block = 0;
+    // device address of the no-allocation entry point's scratch array on this
+    // device (nullptr if the lookup failed, with the reason kept)
+    void *noalloc_scratch = nullptr;
+    size_t noalloc_scratch_bytes = 0;
+    char noalloc_setup_error[160] = {};
 };
 
+// Look up this device's copy of the no-allocation scratch array (vadv.hip);
+// called once per device at set-up, with the device current.
+void vadv_noalloc_setup(Context &c);
 
 // Context of the current device; initialises it on first use.  Returns
 // nullptr (with the error set) on failure.

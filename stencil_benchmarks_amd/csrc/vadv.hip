@@ -42,6 +42,14 @@
 #include "vadv_column.h"
 #include "vadv_ws.h"
 
+// (c', d') scratch of the no-allocation entry point sb_vadv_noalloc: the
+// global-scratch solver (variant 2) on a persistent grid sized to this array.
+// Each device the library runs on holds its own copy until the process ends.
+// Host code never uses this name as an address: the per-device address comes
+// from hipGetSymbolAddress at set-up (vadv_noalloc_setup).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
 namespace sb {
 namespace {
 
@@ -502,7 +510,7 @@ hipError_t launch_ws(int block, bool nt, bool fast, int per_cu_limit, uint64_t b
                                        ctx);
 }
 
-// The checked fields and geometry of one vadv call.
+// The checked fields and geometry of one call (sb_vadv, sb_vadv_noalloc).
 template <typename T>
 struct VadvCall {
     Components comps{};
@@ -664,11 +672,76 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
     });
 }
 
+template <int P>
+int launch_vadv_noalloc(int all_components, void *const *f, const sb_geom *g,
+                        const sb_launch *l, double *seconds) {
+    using T = typename Prec<P>::T;
+    using C = typename Prec<P>::C;
+    VadvCall<T> call;
+    if (int status = prepare_vadv<T>(all_components, f, g, call)) return status;
+    SB_REQUIRE(!l || l->flush == 0,
+               "sb_vadv_noalloc: flushing the caches needs a buffer the library allocates; "
+               "launch->flush must be 0");
+    SB_REQUIRE(!l || l->variant == 0 || l->variant == 2,
+               "sb_vadv_noalloc runs the global-scratch solver (variant 0 or 2), got %d",
+               l->variant);
+    Context *ctx = context();
+    if (!ctx) return 1;
+    SB_REQUIRE(ctx->noalloc_scratch, "no-allocation scratch unavailable on device %d: %s",
+               ctx->device, ctx->noalloc_setup_error);
+    if (call.empty) {
+        if (seconds) *seconds = 0.0;
+        return 0;
+    }
+    // one slice of 64 columns x nz levels of (c', d') per workgroup
+    const size_t slice = size_t(64) * size_t(g->nz) * sizeof(CD<C>);
+    SB_REQUIRE(slice <= ctx->noalloc_scratch_bytes,
+               "sb_vadv_noalloc: nz = %lld needs %zu bytes of scratch per 64 columns, more "
+               "than the %zu bytes it holds",
+               (long long)g->nz, slice, ctx->noalloc_scratch_bytes);
+    const uint64_t grid = std::min<uint64_t>(call.blocks, ctx->noalloc_scratch_bytes / slice);
+    VadvArgs a = call.a;
+    a.scratch_cols = int64_t(grid) * 64;
+    CD<C> *gscratch = static_cast<CD<C> *>(ctx->noalloc_scratch);
+    const int depth = (l && l->unroll) ? l->unroll : (g->nz >= 128 ? 8 : 4);
+    SB_REQUIRE(depth == 4 || depth == 8 || depth == 16, "vadv prefetch depth %d not in {4, 8, 16}",
+               depth);
+    const int flags = (l && l->bz > 0) ? (l->bz & 1) : 0;
+    const Components &comps = call.comps;
+    const T *wcon = call.wcon;
+    return timed_launch(l, seconds, [&]() -> hipError_t {
+        switch (depth) {
+            case 4: return launch_depth<4>(2, flags, grid, 0, comps, wcon, gscratch, a, ctx);
+            case 16: return launch_depth<16>(2, flags, grid, 0, comps, wcon, gscratch, a, ctx);
+            default: return launch_depth<8>(2, flags, grid, 0, comps, wcon, gscratch, a, ctx);
+        }
+    });
+}
+
 }  // namespace
+
+void vadv_noalloc_setup(Context &c) {
+    void *address = nullptr;
+    const hipError_t status = hipGetSymbolAddress(&address, HIP_SYMBOL(scratch));
+    if (status != hipSuccess) {
+        std::snprintf(c.noalloc_setup_error, sizeof(c.noalloc_setup_error),
+                      "hipGetSymbolAddress failed: %s", hipGetErrorString(status));
+        (void)hipGetLastError();
+        return;
+    }
+    c.noalloc_scratch = address;
+    c.noalloc_scratch_bytes = kNoAllocScratchBytes;
+}
 
 }  // namespace sb
 
 using namespace sb;
+
+extern "C" int sb_vadv_noalloc(int prec, int all_components, void *const *fields,
+                               const sb_geom *g, const sb_launch *l, double *seconds) {
+    if (check_geom(g, 1, all_components ? 1 : 0, 0)) return 2;
+    return SB_DISPATCH_PREC(prec, launch_vadv_noalloc, all_components, fields, g, l, seconds);
+}
 
 extern "C" int sb_vadv(int prec, int all_components, void *const *fields, const sb_geom *g,
                        const sb_launch *l, double *seconds) {

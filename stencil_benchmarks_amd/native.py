@@ -92,6 +92,7 @@ SIGNATURES = {
     "sb_laplacian": [_I, _I, _I, _I, _P, _P, _GP, _LP, _DP],
     "sb_hdiff": [_I, _P, _P, _P, _GP, _LP, _DP],
     "sb_vadv": [_I, _I, ctypes.POINTER(_P), _GP, _LP, _DP],
+    "sb_vadv_noalloc": [_I, _I, ctypes.POINTER(_P), _GP, _LP, _DP],
     "sb_stream_fill": [_I, _P, _SZ, _D],
     "sb_stream_fill_random": [_I, _P, _SZ, ctypes.c_uint64],
     "sb_fill_random_fields": [_I, ctypes.POINTER(_P), _I, _SZ, ctypes.POINTER(ctypes.c_uint64)],
