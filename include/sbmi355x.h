@@ -266,9 +266,11 @@ typedef struct sb_subdomain {
     int32_t reserved;
 } sb_subdomain;
 
-/* One timestep for `count` subdomains (two persistent launches: hdiff tiles
- * from a device work queue, vadv column groups per subdomain).  Does not
- * flip the parities; the caller does. */
+/* One timestep for `count` subdomains: a counter reset, one persistent hdiff
+ * launch (tiles from per-XCD device work queues) and one or two persistent
+ * vadv launches (warp-specialised: column-group items dealt to the
+ * workgroups by the host, one launch per LDS class; variant 3: items from a
+ * device counter).  Does not flip the parities; the caller does. */
 int sb_batch_step(int prec, const sb_subdomain *subs, int count, const sb_launch *hdiff_launch,
                   const sb_launch *vadv_launch, double *seconds);
 /* A fixed set of subdomains advanced `steps` steps per run: device tables

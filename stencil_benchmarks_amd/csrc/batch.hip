@@ -7,19 +7,22 @@
 // and the caller swaps u and u_next (parity).  The halo of u / u_next is
 // never written: it is a fixed boundary, identical in both buffers.
 //
-// Each step is two persistent launches (grid = resident workgroups) that
-// draw work from device queues: an atomic counter hands out global item
-// indices, and a workgroup maps an index to (subdomain, local item) by a
-// binary search over per-subdomain prefix sums.  Subdomains are ordered by
-// cost, largest first, so the tail of the queue holds the small items.
+// Each step is a counter-reset kernel, one persistent hdiff launch and one
+// or two persistent vadv launches (grids = resident workgroups).  Subdomains
+// are ordered by cost, largest first, so the tail of each queue holds the
+// small items.
 //   hdiff items: (subdomain, 64 x TY tile, KC-plane chunk), the tile worker
-//     of the plain kernel (hdiff_tile.h).  The items are split into one
-//     contiguous range per XCD, each with its own counter: a workgroup
-//     draws from its XCD's range, then from the others' until all are
-//     done.  Adjacent tiles then run on one XCD, whose L2 fetches their
-//     shared halo rows and row-edge cache lines once; with one queue for
-//     the chip they ran on any XCD (measured on MI355X, 96-subdomain batch:
-//     reads 1.38x / 1.63x the algorithmic bytes in FP64 / float32 storage);
+//     of the plain kernel (hdiff_tile.h), drawn from device counters.  The
+//     items are split into one contiguous range per XCD, each with its own
+//     counter: a workgroup draws from its XCD's range, then from the
+//     others' until all are done.  A ticket maps to its subdomain through
+//     the per-subdomain prefix sums; the workgroup keeps the item range of
+//     its last subdomain, so only a ticket outside it needs the binary
+//     search, and it draws the next ticket while the current item runs.
+//     Adjacent tiles then run on one XCD, whose L2 fetches their shared
+//     halo rows and row-edge cache lines once; with one queue for the chip
+//     they ran on any XCD (measured on MI355X, 96-subdomain batch: reads
+//     1.38x / 1.63x the algorithmic bytes in FP64 / float32 storage);
 //   vadv items:  (subdomain, residue r): the 64-column groups r, r + m,
 //     r + 2m, ... of one subdomain, where m (the subdomain's item count) is
 //     at most the number of resident workgroups, solved by the
