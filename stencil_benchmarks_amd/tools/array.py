@@ -120,11 +120,12 @@ def alloc_array(
     alloc = alloc_smallpages if alloc is None else alloc
 
     if tuple(sorted(layout)) != tuple(range(ndim)):
-        raise ValueError("invalid layout specification")
+        raise ValueError(f"layout {layout} is not a permutation of the {ndim} axes")
     if alignment < 0:
-        raise ValueError("alignment must be non-negative")
+        raise ValueError(f"alignment of {alignment} bytes is negative")
     if not ndim == len(layout) == len(index_to_align):
-        raise ValueError("dimension mismatch")
+        raise ValueError(f"shape, layout and index_to_align differ in length "
+                         f"({ndim}, {len(layout)}, {len(index_to_align)})")
 
     strides, span = compute_strides(shape, dtype.itemsize, layout, alignment)
 
@@ -132,11 +133,13 @@ def alloc_array(
     if apply_offset:
         sets = l1_dcache_sets()
         if not sets:
-            warnings.warn("could not determine number of cache sets, assuming 64")
+            warnings.warn("L1 data cache set count unknown on this host; staggering "
+                          "allocations over 64 sets")
             sets = 64
         line = l1_dcache_linesize()
         if not line:
-            warnings.warn("could not determine cache line size, assuming 64B")
+            warnings.warn("L1 data cache line size unknown on this host; staggering "
+                          "allocations by 64 bytes")
             line = 64
         offset = (_stagger_count % sets) << _floor_log2(max(line, alignment))
         _stagger_count += 1

@@ -1,8 +1,10 @@
 """Result verification with the reference's tolerances.
 
-Follows stencil_benchmarks/tools/validation.py:37-116: ``np.isclose`` with
-``rtol=1e-4, atol=1e-5`` for float32 and numpy's defaults otherwise; a
-failure raises :class:`ValidationError` listing the failing points.
+The acceptance rule is the reference's (stencil_benchmarks/tools/validation.py:
+98-116): ``np.isclose`` with ``rtol=1e-4, atol=1e-5`` for float32 and numpy's
+defaults otherwise.  A failure raises :class:`ValidationError` naming the
+field, the count of failing points, the largest absolute and relative
+differences and the worst points.
 :func:`check_close` adds an explicit-tolerance form for the stricter checks
 this backend states (bit-exact FP64, FP32 storage bounds).
 """
@@ -20,27 +22,25 @@ def _tolerances(dtype):
     return dict()
 
 
-def _report(result, expected, failure, limit=20):
+def _raise(name, result, expected, failure, shown=10):
+    """ValidationError naming the field, the number of failing points, the
+    largest differences and the `shown` worst points (index: got / expected)."""
     where = np.argwhere(failure)
-    lines = []
-    for index in where[:limit]:
-        idx = tuple(int(v) for v in index)
-        lines.append(
-            "failed at "
-            + ", ".join(f"{v:3}" for v in idx)
-            + f": {result[idx]:12.7f}    != {expected[idx]:12.7f}"
-        )
-    if len(where) > limit:
-        lines.append(f"omitted further {len(where) - limit} failures.")
-    return "\n".join(lines)
-
-
-def _raise(name, result, expected, failure):
-    raise ValidationError(
-        f"validation of field {name} failed at "
-        f"{int(np.count_nonzero(failure))} points:\n"
-        + _report(result, expected, failure)
-    )
+    got = np.asarray(result, dtype=np.float64)[failure]
+    want = np.asarray(expected, dtype=np.float64)[failure]
+    diff = np.abs(got - want)
+    diff = np.where(np.isnan(diff), np.inf, diff)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rel = diff / np.abs(want)
+    worst = np.argsort(-diff, kind="stable")[:shown]
+    lines = [f"{name}: {len(where)} of {failure.size} points differ "
+             f"(largest |difference| {diff.max():.4g}, largest relative {np.nanmax(rel):.4g})"]
+    for n in worst:
+        index = ", ".join(str(int(v)) for v in where[n])
+        lines.append(f"  [{index}]: {float(got[n])!r} / expected {float(want[n])!r}")
+    if len(where) > shown:
+        lines.append(f"  ... and {len(where) - shown} more")
+    raise ValidationError("\n".join(lines))
 
 
 def check_equality(name, result, expected):

@@ -58,7 +58,7 @@ def test_huge_pages_transparent():
 def test_validation_tolerances():
     a = np.ones((4, 4, 4))
     validation.check_equality("a", a, a * (1 + 1e-6))
-    with pytest.raises(validation.ValidationError, match="1 points"):
+    with pytest.raises(validation.ValidationError, match=r"a: 1 of 64 points differ"):
         b = a.copy()
         b[1, 2, 3] = 2
         validation.check_equality("a", b, a)
