@@ -436,12 +436,28 @@ def numpy_config1(budget_s=5.0):
             "threads": 1, "note": "numpy elementwise kernels run on one thread"}
 
 
+def sources_sha(paths):
+    """sha256 over the named source files (relative to the repository)."""
+    import hashlib
+
+    digest = hashlib.sha256()
+    for rel in paths:
+        digest.update(rel.encode())
+        digest.update((ROOT / rel).read_bytes())
+    return digest.hexdigest()
+
+
 def profiled_traffic(key):
     """HBM traffic per launch measured by rocprofv3 FETCH_SIZE/WRITE_SIZE runs
-    (scaled by the copy-kernel calibration), committed in profiles/traffic.json."""
-    path = Path(__file__).resolve().parent / "profiles" / "traffic.json"
+    (scaled by the copy-kernel calibration), committed in profiles/traffic.json;
+    None unless the entry was measured on the kernel sources of this tree."""
+    path = ROOT / "profiles" / "traffic.json"
     try:
-        return json.loads(path.read_text())[key]["traffic_bytes"]
+        entry = json.loads(path.read_text())[key]
+        if sources_sha(entry["sources"]) != entry["sources_sha256"]:
+            log(f"profiles/traffic.json entry {key} was measured on other kernel sources")
+            return None
+        return entry["traffic_bytes"]
     except (OSError, KeyError, ValueError):
         return None
 
@@ -456,7 +472,7 @@ def main():
     parser.add_argument("--warmup", type=int, default=60)
     parser.add_argument("--quick", action="store_true", help="headline only")
     parser.add_argument("--only", choices=["hdiff", "hdiff_f32", "vadv", "laplacian", "stream",
-                                           "batched", "serving"],
+                                           "stream_copy", "batched", "serving"],
                         help="run a single workload (profiling)")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     args = parser.parse_args()
@@ -612,6 +628,9 @@ def run_single(args):
     elif args.only == "stream":
         out["stream_gbs"], out["stream_best_shapes"], table = bench_stream(3)
         out["stream_sweep"] = table
+    elif args.only == "stream_copy":
+        # one plain shape (counter calibration: bytes read and written are known)
+        out["stream_gbs"], out["stream_best_shapes"], _ = bench_stream(3, [(256, 2, 1, 0, 0)])
     elif args.only == "batched":
         out["batched_f64"] = bench_batched(np.float64, np.float64)
         out["batched_f32s_f64c"] = bench_batched(np.float64, np.float32)

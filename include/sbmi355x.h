@@ -177,14 +177,19 @@ int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
  * *tensstage fields are written; ccol, dcol and datacol are never touched
  * (the Thomas coefficients stay on chip, or in a library-owned scratch for
  * variant 2).  Requires hi >= 1 (all components: hj >= 1 too).  Variants:
- * 0 default (4 for nz >= 96, or nz >= 64 with float32 storage, with at
- * least two column groups per resident workgroup, else 3; 2 beyond the LDS
- * limit of nz = 160), 1 one wave per
+ * 0 default (5 where nz <= 96 and every 64-column group is in flight at
+ * once; else 4 for nz >= 96, or nz >= 64 with float32 storage, with at
+ * least two column groups per resident workgroup, else 3; beyond the LDS
+ * limit of those (nz > 160 with FP64 arithmetic) 5 up to nz = 201, then 2),
+ * 1 one wave per
  * group with the coefficients in LDS, 2 coefficients in a global FP64
  * scratch, 3 single-wave pipeline fusing the forward sweep of one group with
  * the backward of the previous, 4 warp-specialised solver (four waves share
  * one LDS slot buffer; launch->unroll = levels per barrier step, 4 or 8,
- * 0 = 4, when variant 4 is asked for explicitly).  launch->bz: bit 0
+ * 0 = 4, when variant 4 is asked for explicitly), 5 one wave per group with
+ * the coefficients of the lowest 40 levels (80 with float32 arithmetic) in
+ * registers and the rest in LDS (nz <= 201 with FP64 arithmetic).
+ * launch->bz: bit 0
  * non-temporal streamed loads, bit 2 (variant 4, SB_F32_F64 only) reciprocal
  * estimate plus Newton steps instead of the correctly rounded division; 0
  * and -1 plain loads. */

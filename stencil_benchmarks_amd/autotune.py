@@ -66,7 +66,9 @@ PLAIN_HDIFF_SPACE = tuple({"variant": 1, "by": by, "bz": bz, "unroll": u}
 # variant 2 (global scratch), the one solver for columns beyond the LDS
 PLAIN_VADV_SPACE = tuple({"variant": v, "unroll": u, "bz": bz}
                          for v, depths in ((4, (4, 8)), (3, (4, 8, 16)))
-                         for u in depths for bz in (1, -1)) + ({"variant": 2, "unroll": 8, "bz": 1},)
+                         for u in depths for bz in (1, -1)) + (
+    {"variant": 5, "unroll": 0, "bz": -1}, {"variant": 5, "unroll": 0, "bz": 1},
+    {"variant": 2, "unroll": 8, "bz": 1})
 
 
 def default_cache_path():

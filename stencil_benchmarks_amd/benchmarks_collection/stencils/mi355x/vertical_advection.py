@@ -2,13 +2,15 @@
 
 One lane per (i, j) column with contiguous i, forward sweep and fused
 backward substitution in one kernel (csrc/vadv.hip).  ``Fused`` picks the
-solver: the warp-specialised one (four waves per 64-column group) for
-nz >= 96 (nz >= 64 with float32 storage), the single-wave pipeline below
-that, both with the Thomas
-coefficients in LDS (nz <= 160 in FP64 arithmetic), and a global FP64
-scratch beyond.  ``LdsColumn``, ``GlobalScratch``, ``Pipelined`` and
-``WarpSpecialised`` pin one solver (the last with ``levels_per_step`` 4 or
-8 per workgroup barrier).  Arithmetic follows
+solver: one wave per 64-column group with the lowest levels' Thomas
+coefficients in registers where every group of the domain fits on the chip
+at once (nz <= 96), else the warp-specialised one (four waves per group)
+for nz >= 96 (nz >= 64 with float32 storage) and the single-wave pipeline
+below that, both with the Thomas coefficients in LDS (nz <= 160 in FP64
+arithmetic), and a global FP64 scratch beyond.  ``LdsColumn``,
+``GlobalScratch``, ``Pipelined``, ``WarpSpecialised`` and
+``RegisterColumns`` pin one solver (``WarpSpecialised`` with
+``levels_per_step`` 4 or 8 per workgroup barrier).  Arithmetic follows
 stencils/base.py:415-473 (bit-exact in FP64; with float32 storage the
 default arithmetic is FP64 throughout).
 """
@@ -76,3 +78,7 @@ class WarpSpecialised(VerticalAdvectionMixin, VerticalAdvectionStencil):
         options = super().launch_options()
         options.unroll = self.levels_per_step
         return options
+
+
+class RegisterColumns(VerticalAdvectionMixin, VerticalAdvectionStencil):
+    VARIANT = 5

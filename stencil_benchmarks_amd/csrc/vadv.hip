@@ -40,6 +40,7 @@
 #include <algorithm>
 
 #include "vadv_column.h"
+#include "vadv_regs.h"
 #include "vadv_ws.h"
 
 // (c', d') scratch of the no-allocation entry point sb_vadv_noalloc: the
@@ -374,6 +375,56 @@ __global__ __launch_bounds__(64 * kWsWaves) void vadv_ws_kernel(Components comps
     }
 }
 
+// Variant 5: one wave per 64-column group, (c', d') of the lowest levels in
+// registers and of the rest in LDS (vadv_regs.h); consecutive groups on one
+// XCD.
+template <typename C>
+constexpr int hybrid_nr() { return sizeof(C) == 8 ? 40 : 80; }
+// levels of loads in flight: 8 levels of six loads stay below the 63
+// outstanding vector-memory instructions a wave can count (20 levels measured
+// 8 % slower at 256 x 256 x 80 with float32 storage)
+template <typename T>
+constexpr int hybrid_pf() { return 8; }
+
+template <typename T, typename C, bool NT>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void vadv_hybrid_kernel(Components comps, const T *__restrict__ wcon, PipeArgs a) {
+    extern __shared__ unsigned char lds_raw[];
+    constexpr int AUX = NT ? 2 : 0;
+    CD<C> *lds = reinterpret_cast<CD<C> *>(lds_raw) + (threadIdx.x & 63);
+    const int64_t g = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
+    if (g >= a.ngroups) return;
+    for (int c = 0; c < a.ncomp; ++c) {
+        const Component &cp = comps.c[c];
+        const Rsrc R{make_rsrc(cp.stage, a.span_bytes), make_rsrc(cp.pos, a.span_bytes),
+                     make_rsrc(cp.tens, a.span_bytes), make_rsrc(cp.tensstage, a.span_bytes),
+                     make_rsrc(wcon, a.span_bytes)};
+        HybridColumn<T, C, hybrid_nr<C>(), hybrid_pf<T>(), AUX>::solve(R, cp.wshift, a, g, lds);
+    }
+}
+
+template <typename T, typename C>
+size_t hybrid_lds_bytes(int64_t nz) {
+    return size_t(hybrid_lds_rows(nz, hybrid_nr<C>(), hybrid_pf<T>())) * 64 * sizeof(CD<C>);
+}
+
+template <typename T, typename C>
+hipError_t launch_hybrid(bool nt, uint64_t groups, size_t lds_bytes, const Components &comps,
+                         const T *wcon, const PipeArgs &pa, Context *ctx) {
+    PipeArgs args = pa;
+    args.xcds = groups >= uint64_t(ctx->xcds) ? uint32_t(ctx->xcds) : 1u;
+    const void *fn = nt ? reinterpret_cast<const void *>(&vadv_hybrid_kernel<T, C, true>)
+                        : reinterpret_cast<const void *>(&vadv_hybrid_kernel<T, C, false>);
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
+    if (nt)
+        enqueue_kernel((vadv_hybrid_kernel<T, C, true>), dim3(unsigned(groups)), dim3(64),
+                       lds_bytes, ctx->stream, comps, wcon, args);
+    else
+        enqueue_kernel((vadv_hybrid_kernel<T, C, false>), dim3(unsigned(groups)), dim3(64),
+                       lds_bytes, ctx->stream, comps, wcon, args);
+    return hipGetLastError();
+}
+
 int ensure_scratch(Context *ctx, void **scratch, size_t *have, size_t bytes) {
     if (*have >= bytes) return 0;
     if (*scratch) {
@@ -589,8 +640,44 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
     const bool ws_default = ws_fits && buffer_ok &&
                             uint64_t(last_elem + 1 + g->sk) * sizeof(T) < kWsDropOffset &&
                             g->nz >= ws_min_nz && groups >= 2 * int64_t(ctx->compute_units);
-    if (variant == 0) variant = ws_default ? 4 : lds_fits ? (buffer_ok ? 3 : 1) : 2;
-    SB_REQUIRE(variant >= 1 && variant <= 4, "vadv variant %d not available", variant);
+    // variant 5 (one wave per group, no pipeline fill or drain) where every
+    // group is in flight at once: at most one wave per SIMD (its registers)
+    // and as many per CU as its LDS allows (measured on MI355X, 256x256x80:
+    // float32 storage 34.6 against 39.6 us, FP64 53.5 against 54.7 us;
+    // with more groups than that, as at 384x384x80, it ran 8-20 % slower)
+    const size_t v5_lds = hybrid_lds_bytes<T, C>(g->nz);
+    const int64_t v5_per_cu =
+        v5_lds ? std::min<int64_t>(4, int64_t(ctx->lds_per_block / v5_lds)) : 4;
+    const bool v5_default = buffer_ok && g->nz <= 96 && v5_per_cu > 0 &&
+                            uint64_t(last_elem + 1 + g->sk) * sizeof(T) < kRegDropOffset &&
+                            groups <= v5_per_cu * int64_t(ctx->compute_units);
+    // beyond the LDS of variants 1, 3 and 4 (nz > 160 with FP64 arithmetic)
+    // variant 5 still holds the column while its LDS part fits (nz <= 201),
+    // ahead of the global scratch (512x512x160 FP64: 765 against 863 us)
+    const bool v5_fits = buffer_ok && v5_lds <= ctx->lds_per_block &&
+                         uint64_t(last_elem + 1 + g->sk) * sizeof(T) < kRegDropOffset;
+    if (variant == 0)
+        variant = v5_default ? 5
+                  : ws_default ? 4
+                  : lds_fits   ? (buffer_ok ? 3 : 1)
+                  : v5_fits    ? 5
+                               : 2;
+    SB_REQUIRE(variant >= 1 && variant <= 5, "vadv variant %d not available", variant);
+    if (variant == 5) {
+        const size_t hlds = hybrid_lds_bytes<T, C>(g->nz);
+        SB_REQUIRE(hlds <= ctx->lds_per_block,
+                   "nz = %lld needs %zu bytes of LDS per column group, device allows %zu",
+                   (long long)g->nz, hlds, ctx->lds_per_block);
+        SB_REQUIRE(buffer_ok && uint64_t(last_elem + 1 + g->sk) * sizeof(T) < kRegDropOffset,
+                   "vadv variant 5 needs positive strides and fields below 4 GiB");
+        SB_REQUIRE(uint64_t(groups) < (uint64_t(1) << 31), "domain too large for one launch");
+        PipeArgs pa{a.nx, a.ny, a.nz, a.si, a.sj, a.sk, a.ncomp, groups,
+                    uint32_t(uint64_t(last_elem + 1) * sizeof(T)), uint32_t(ctx->xcds)};
+        const bool nt = l && l->bz > 0 && (l->bz & 1);
+        return timed_launch(l, seconds, [&]() -> hipError_t {
+            return launch_hybrid<T, C>(nt, uint64_t(groups), hlds, comps, wcon, pa, ctx);
+        });
+    }
     SB_REQUIRE(variant == 2 || variant == 4 || lds_fits,
                "nz = %lld needs %zu bytes of LDS per workgroup, device allows %zu",
                (long long)g->nz, lds_bytes + kStaticLds, ctx->lds_per_block);

@@ -204,7 +204,7 @@ def _vadv_expected(fields, domain, halo, prec, components):
 
 
 @pytest.mark.parametrize("prec", PRECISIONS)
-@pytest.mark.parametrize("variant", [1, 2, 3, 4])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("all_components", [False, True])
 @pytest.mark.parametrize("domain", VADV_DOMAINS)
 def test_vadv(gpu, prec, variant, all_components, domain):
@@ -256,6 +256,36 @@ def test_vadv_warp_specialised(gpu, prec, domain, nt, levels):
                      extra=dict(bz=nt, unroll=levels))
     expected = _expected(lambda *f: oracle.vadv(*f, domain, halo), fields[:5], prec)
     assert np.array_equal(got[inner(domain, halo)], expected)
+
+
+# nz around the register part of the one-wave solver (40 levels with FP64
+# arithmetic, 80 with float32) and its LDS blocks (8 levels), the register
+# part alone, and the LDS limit (nz = 201 with FP64 arithmetic)
+HYBRID_DOMAINS = [(70, 37, 2), (65, 9, 3), (64, 20, 39), (64, 20, 40), (64, 20, 41),
+                  (96, 30, 42), (130, 40, 48), (33, 17, 49), (200, 60, 80), (64, 70, 81),
+                  (70, 11, 97), (64, 64, 160), (65, 3, 201)]
+
+
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("domain", HYBRID_DOMAINS)
+@pytest.mark.parametrize("nt", [0, 1])
+def test_vadv_one_wave_register_columns(gpu, prec, domain, nt):
+    halo = (1, 0, 0)
+    dtype = _dtype(prec)
+    fields = make_fields(domain, halo, dtype, count=8, seed=3 * sum(domain))
+    got, = _run_vadv(gpu, fields, [3], domain, halo, _prec(gpu, prec), False, 5,
+                     extra=dict(bz=nt))
+    expected = _expected(lambda *f: oracle.vadv(*f, domain, halo), fields[:5], prec)
+    assert np.array_equal(got[inner(domain, halo)], expected)
+
+
+def test_vadv_one_wave_rejects_columns_beyond_lds(gpu):
+    from stencil_benchmarks_amd.benchmark import ExecutionError
+
+    domain, halo = (8, 2, 202), (1, 0, 0)
+    fields = make_fields(domain, halo, np.float64, count=8)
+    with pytest.raises(ExecutionError, match="LDS"):
+        _run_vadv(gpu, fields, [3], domain, halo, gpu.SB_F64, False, 5)
 
 
 def test_vadv_warp_specialised_rejects_other_step_sizes(gpu):
@@ -328,10 +358,12 @@ def _run_vadv(native, fields, written, domain, halo, code, all_components, varia
     return results
 
 
-def test_vadv_tall_column_uses_global_scratch(gpu):
-    # nz = 161 exceeds the LDS budget of the FP64 column kernel; variant 0
-    # must fall back to the global scratch and still match.
-    domain, halo = (67, 2, 161), (1, 0, 0)
+@pytest.mark.parametrize("nz", [161, 201, 202, 300])
+def test_vadv_tall_columns_default(gpu, nz):
+    # nz > 160 exceeds the LDS of the FP64 LDS-only solvers; variant 0 takes
+    # the one-wave register + LDS solver up to nz = 201 and the global
+    # scratch beyond, and must still match
+    domain, halo = (67, 2, nz), (1, 0, 0)
     fields = make_fields(domain, halo, np.float64, count=8, seed=3)
     got, = _run_vadv(gpu, fields, [3], domain, halo, gpu.SB_F64, False, 0)
     expected = oracle.vadv(*fields[:5], domain, halo)
