@@ -256,7 +256,7 @@ int sb_stream_check(int prec, const void *x, size_t n, double expected,
  * Launch options: hdiff_launch->by / ->bz / ->unroll = tile height / planes
  * per item / non-temporal flags (as sb_hdiff); vadv_launch->variant = 4
  * (default: warp-specialised solver, items of ->by column groups, 0 = 16,
- * dealt to the workgroups by the host in order of decreasing cost, ->unroll
+ * drawn from per-XCD device work queues in order of decreasing cost, ->unroll
  * = levels per barrier step 4 or 8, 0 = 4, ->bx = waves per workgroup 4 or
  * 5, 0 = 4, the fifth loading wcon for the others) or 3
  * (single-wave pipeline, items from a device counter; ->unroll = prefetch
@@ -273,9 +273,9 @@ typedef struct sb_subdomain {
 
 /* One timestep for `count` subdomains: a counter reset, one persistent hdiff
  * launch (tiles from per-XCD device work queues) and one or two persistent
- * vadv launches (warp-specialised: column-group items dealt to the
- * workgroups by the host, one launch per LDS class; variant 3: items from a
- * device counter).  Does not flip the parities; the caller does. */
+ * vadv launches (warp-specialised: column-group items from per-XCD device
+ * work queues, one launch per LDS class; variant 3: items from one device
+ * counter).  Does not flip the parities; the caller does. */
 int sb_batch_step(int prec, const sb_subdomain *subs, int count, const sb_launch *hdiff_launch,
                   const sb_launch *vadv_launch, double *seconds);
 /* A fixed set of subdomains advanced `steps` steps per run: device tables

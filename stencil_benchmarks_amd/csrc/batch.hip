@@ -7,8 +7,9 @@
 // and the caller swaps u and u_next (parity).  The halo of u / u_next is
 // never written: it is a fixed boundary, identical in both buffers.
 //
-// Each step is a counter-reset kernel, one persistent hdiff launch and one
-// or two persistent vadv launches (grids = resident workgroups).  Subdomains
+// A sequence of steps starts with one counter-reset kernel; each step is one
+// persistent hdiff launch and one or two persistent vadv launches (grids =
+// resident workgroups), all drawing items from device work queues.  Subdomains
 // are ordered by cost, largest first, so the tail of each queue holds the
 // small items.
 //   hdiff items: (subdomain, 64 x TY tile, KC-plane chunk), the tile worker
@@ -28,20 +29,20 @@
 //     at most the number of resident workgroups, solved by the
 //     warp-specialised column solver of the plain kernel (vadv_ws.h; the
 //     single-wave pipeline of vadv_column.h with variant 3) with an LDS
-//     buffer sized for the tallest column in the batch.  The warp-specialised
-//     kernel takes no tickets (its LDS is full at nz = 160, which leaves no
-//     room to broadcast one): the host deals its items to the workgroups
-//     ahead of time (snake order over subdomains sorted by item cost, from a
-//     cost model of the solver's pipeline, (groups + 1) x (levels / 4 + 3)
-//     barrier steps per item), and every workgroup walks its own list.
+//     buffer sized for the tallest column of its launch.  The host orders
+//     the items (subdomains by decreasing item cost from a model of the
+//     solver's pipeline, (groups + 1) x (levels / M + 3) barrier steps per
+//     item, residues in order) and cuts the list into one contiguous range
+//     of about equal cost per XCD; workgroups draw tickets from their XCD's
+//     range, then from the others', like the hdiff items.  Consecutive
+//     residues (adjacent groups of a row) then run on one XCD at about the
+//     same time, so the waves of the chip touch the same few k planes (and
+//     pages) at a time, as in the plain kernel; runs of consecutive groups
+//     per workgroup scatter the waves over many planes and ran 3x slower.
 //     Subdomains whose slot buffer fits twice in a CU's LDS (nz <= 80 with
 //     FP64 arithmetic) run in a launch of their own at two workgroups per
 //     CU, the taller ones at one (measured on MI355X: 28 % faster for
-//     float32-storage columns of nz <= 80).  Workgroups that
-//     take consecutive items of one subdomain walk adjacent groups level by
-//     level, so the waves of the chip touch the same few k planes (and
-//     pages) at a time, as in the plain kernel; runs of consecutive groups
-//     per workgroup scatter the waves over many planes and ran 3x slower.
+//     float32-storage columns of nz <= 80).
 // The counters are cleared by a small kernel at the start of every
 // sequence (no memset nodes, so the sequence can be captured in a
 // hipGraph and replayed).  Every workgroup exits once the counters pass
@@ -65,7 +66,12 @@ constexpr int kMinGroupsPerItem = 8;  // amortises the pipeline's fill and drain
 // hdiff work queues per step (one per XCD, at most this many XCDs), then
 // the vadv counter: counters per step
 constexpr int kHdiffQueues = 8;
-constexpr int kCountersPerStep = kHdiffQueues + 1;
+// warp-specialised vadv work queues per LDS class (one per XCD), then the
+// counter of the single-wave pipeline (variant 3)
+constexpr int kVadvQueues = 8;
+constexpr int kVadvCounters = kHdiffQueues;
+constexpr int kPipeCounter = kHdiffQueues + 2 * kVadvQueues;
+constexpr int kCountersPerStep = kPipeCounter + 1;
 // Groups per item of the warp-specialised solver: its fill and drain cost
 // nb + 3 steps per item; with the host's cost-ordered schedule, larger items
 // do not unbalance the workgroups (16 measured best on the 96-subdomain
@@ -214,33 +220,51 @@ void vadv_batch_kernel(const SubDev *__restrict__ subs, const int64_t *__restric
     }
 }
 
-// Workgroup w walks items list[off[w]] .. list[off[w+1]-1] (the host's
-// schedule); w is the logical worker of this workgroup, numbered so that
-// consecutive workers share an XCD: the schedule hands consecutive items of
-// a subdomain (adjacent groups of a row, which share the cache lines at
-// their edges) to consecutive workers.  MB levels per barrier step, NW
-// waves per workgroup.
+// Items of one LDS class in `queues` ranges off[q] .. off[q+1]-1 (one per
+// XCD, in the host's order), each with its own counter: a workgroup draws
+// from range blockIdx.x % queues (its XCD: workgroups are dispatched to the
+// XCDs round-robin), then from the others until all are done.  The host
+// puts consecutive residues of a subdomain (adjacent groups of a row, which
+// share the cache lines at their edges) into one range, next to each other.
+// The slot buffer fills the LDS at nz = 160, so the ticket is broadcast
+// through its first word while no item is in flight.  MB levels per
+// barrier step, NW waves per workgroup.
 template <typename T, typename C, bool FAST, int MB, int NW>
 __global__ __launch_bounds__(64 * NW) void vadv_ws_batch_kernel(
     const SubDev *__restrict__ subs, const int32_t *__restrict__ off,
-    const VItem *__restrict__ list, int xcds) {
+    const VItem *__restrict__ list, unsigned long long *__restrict__ counters, int queues) {
     extern __shared__ unsigned char lds_raw[];
     C *lane0 = reinterpret_cast<C *>(lds_raw) + (threadIdx.x & 63);
-    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x, uint32_t(xcds));
-    const int32_t end = off[w + 1];
-    for (int32_t i = off[w]; i < end; ++i) {
-        const VItem it = list[i];
-        const int s = int(__builtin_amdgcn_readfirstlane(it.rank));
-        const SubDev d = subs[s];
-        const int64_t groups = vadv_groups(d.nx, d.ny);
-        const int64_t m = d.vstride;
-        const int64_t r = int64_t(__builtin_amdgcn_readfirstlane(it.residue));
-        const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, 1, groups, d.span_bytes,
-                         uint32_t(xcds)};
-        const WsFields f{d.out, d.pos, d.tens, d.ts, d.wcon, d.span_bytes};
-        // u component: wcon neighbour at i+1
-        WsSolver<T, C, 0, FAST, MB, NW>::run(f, d.si, a, WsSeq{r, m, (groups - r + m - 1) / m},
-                                             lane0, lane0 + ws_rows(d.nz, MB) * 64);
+    long long *s_ticket = reinterpret_cast<long long *>(lds_raw);
+    const int home = int(blockIdx.x % uint32_t(queues));
+    for (int r = 0; r < queues; ++r) {
+        const int q = home + r < queues ? home + r : home + r - queues;
+        const int32_t lo = off[q], hi = off[q + 1];
+        // the next ticket is drawn while the current item runs; every ticket
+        // drawn is either worked or ends this range
+        unsigned long long next = 0;
+        if (threadIdx.x == 0) next = atomicAdd(counters + q, 1ull);
+        for (;;) {
+            __syncthreads();  // the previous item's LDS accesses are done
+            if (threadIdx.x == 0) *s_ticket = (long long)lo + (long long)next;
+            __syncthreads();
+            const int64_t i = uniform64(uint64_t(*s_ticket));
+            __syncthreads();
+            if (i >= hi) break;
+            if (threadIdx.x == 0) next = atomicAdd(counters + q, 1ull);
+            const VItem it = list[i];
+            const int s = int(__builtin_amdgcn_readfirstlane(it.rank));
+            const SubDev d = subs[s];
+            const int64_t groups = vadv_groups(d.nx, d.ny);
+            const int64_t m = d.vstride;
+            const int64_t res = int64_t(__builtin_amdgcn_readfirstlane(it.residue));
+            const PipeArgs a{d.nx, d.ny, d.nz, d.si, d.sj, d.sk, 1, groups, d.span_bytes,
+                             uint32_t(queues)};
+            const WsFields f{d.out, d.pos, d.tens, d.ts, d.wcon, d.span_bytes};
+            // u component: wcon neighbour at i+1
+            WsSolver<T, C, 0, FAST, MB, NW>::run(f, d.si, a, WsSeq{res, m, (groups - res + m - 1) / m},
+                                                 lane0, lane0 + ws_rows(d.nz, MB) * 64);
+        }
     }
 }
 
@@ -262,7 +286,7 @@ struct Plan {
     struct WsClass {
         int max_nz = 0;     // tallest column of the class (sizes its LDS)
         int per_cu = 0;     // workgroups per CU
-        int32_t *off = nullptr;  // schedule: CSR offsets per workgroup, items
+        int32_t *off = nullptr;  // items (decreasing cost) and their per-queue ranges
         VItem *list = nullptr;
         std::vector<int32_t> host_off;
         std::vector<VItem> host_list;
@@ -346,6 +370,9 @@ size_t plan_vadv_lds(int prec, int max_nz, int variant, int block) {
     return prec == SB_F32 ? vadv_lds_bytes<float>(max_nz) : vadv_lds_bytes<double>(max_nz);
 }
 
+// work queues of the warp-specialised vadv: one per XCD
+int vadv_queues(const Context *ctx) { return std::max(1, std::min(ctx->xcds, kVadvQueues)); }
+
 // vadv workgroups of one launch for the tallest column: as many as LDS lets
 // a CU hold (the warp-specialised kernel: at most 2, its VGPR budget)
 unsigned vadv_workers(const Plan &p, const Context *ctx) {
@@ -356,15 +383,17 @@ unsigned vadv_workers(const Plan &p, const Context *ctx) {
     return unsigned(fit * ctx->compute_units);
 }
 
-// Schedule the warp-specialised vadv items onto `workers` workgroups.  Cost
-// of an item of n groups of nb blocks: (n + 1) (nb + skew) barrier steps
-// (the solver's pipeline with its fill and drain).  Subdomains are taken in
-// order of decreasing item cost, the items of one subdomain in residue
-// order, and dealt to the workers in a snake: 0 .. W-1, then W-1 .. 0, and
-// so on.  This balances the workers nearly as well as longest-first onto
-// the least-loaded worker, and keeps consecutive residues of a subdomain
-// (adjacent groups of a row) on adjacent workers, which share an XCD.
-void schedule_ws(Plan &p, int cls, int workers) {
+// Order the warp-specialised vadv items of one LDS class for the device
+// queues.  Cost of an item of n groups of nb blocks: (n + 1) (nb + skew)
+// barrier steps (the solver's pipeline with its fill and drain).  Subdomains
+// are taken in order of decreasing item cost, the items of one subdomain in
+// residue order, and dealt in a snake over `workers` slots (0 .. W-1, then
+// W-1 .. 0, and so on), which balances the slots nearly as well as
+// longest-first onto the least-loaded one; slot w belongs to queue
+// w / (W / queues), and each queue takes its slots' items in dealing order.
+// The first pass so hands the workgroups of an XCD consecutive residues
+// (adjacent groups), and the workgroups then draw the rest dynamically.
+void schedule_ws(Plan &p, int cls, int workers, int queues) {
     std::vector<int64_t> item_cost(size_t(p.count));
     const int block = vadv_ws_block(&p.vl);
     for (int r = 0; r < p.count; ++r) {
@@ -378,25 +407,23 @@ void schedule_ws(Plan &p, int cls, int workers) {
         if (p.host_vclass[size_t(r)] == cls) ranks.push_back(r);
     std::stable_sort(ranks.begin(), ranks.end(),
                      [&](int a, int b) { return item_cost[size_t(a)] > item_cost[size_t(b)]; });
-    std::vector<int32_t> owner;
-    std::vector<VItem> items;
-    std::vector<int32_t> per(size_t(workers), 0);
+    const int64_t per_queue = std::max<int64_t>(1, workers / queues);
+    std::vector<std::vector<VItem>> by_queue(static_cast<size_t>(queues));
     int64_t i = 0;
     for (int r : ranks) {
         for (int32_t res = 0; res < p.host_subs[0][r].vstride; ++res, ++i) {
             const int64_t pass = i / workers, o = i % workers;
-            const int32_t w = int32_t((pass & 1) ? workers - 1 - o : o);
-            items.push_back(VItem{int32_t(r), res});
-            owner.push_back(w);
-            ++per[size_t(w)];
+            const int64_t w = (pass & 1) ? workers - 1 - o : o;
+            by_queue[size_t(std::min<int64_t>(w / per_queue, queues - 1))].push_back(VItem{int32_t(r), res});
         }
     }
     Plan::WsClass &c = p.ws[cls];
-    c.host_off.assign(size_t(workers) + 1, 0);
-    for (int w = 0; w < workers; ++w) c.host_off[w + 1] = c.host_off[w] + per[w];
-    c.host_list.resize(items.size());
-    std::vector<int32_t> fill(c.host_off.begin(), c.host_off.end() - 1);
-    for (size_t k = 0; k < items.size(); ++k) c.host_list[size_t(fill[owner[k]]++)] = items[k];
+    c.host_list.clear();
+    c.host_off.assign(size_t(queues) + 1, 0);
+    for (int q = 0; q < queues; ++q) {
+        c.host_list.insert(c.host_list.end(), by_queue[size_t(q)].begin(), by_queue[size_t(q)].end());
+        c.host_off[size_t(q) + 1] = int32_t(c.host_list.size());
+    }
 }
 
 // Build descriptors and prefix sums (largest subdomains first).
@@ -478,7 +505,8 @@ void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch
     }
     if (ws)
         for (int c = 0; c < 2; ++c)
-            if (p.ws[c].max_nz) schedule_ws(p, c, p.ws[c].per_cu * ctx->compute_units);
+            if (p.ws[c].max_nz)
+                schedule_ws(p, c, p.ws[c].per_cu * ctx->compute_units, vadv_queues(ctx));
 }
 
 // Enqueue one step (both kernels) on ctx->stream.
@@ -516,15 +544,17 @@ hipError_t enqueue_step(const Plan &p, int flip, unsigned long long *counters, C
             const Plan::WsClass &c = p.ws[cls];
             if (!c.max_nz) continue;
             const unsigned grid = unsigned(c.per_cu * ctx->compute_units);
-            // the schedule was built for this grid: one list per workgroup
-            if (c.host_off.size() != size_t(grid) + 1) return hipErrorInvalidValue;
+            // the item ranges were cut for this many queues
+            const int queues = vadv_queues(ctx);
+            if (c.host_off.size() != size_t(queues) + 1) return hipErrorInvalidValue;
             const int block = vadv_ws_block(&p.vl), waves = vadv_ws_waves(&p.vl);
             const size_t wlds = ws_lds_bytes<C>(c.max_nz, block);
             auto launch_w = [&](auto kernel) -> hipError_t {
                 (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                           hipFuncAttributeMaxDynamicSharedMemorySize, int(wlds));
                 hipLaunchKernelGGL(kernel, dim3(grid), dim3(64 * waves), wlds, ctx->stream,
-                                   subs, c.off, c.list, ctx->xcds);
+                                   subs, c.off, c.list, counters + kVadvCounters + cls * kVadvQueues,
+                                   queues);
                 return hipGetLastError();
             };
             // (fast reciprocal, levels per step, waves) -> kernel
@@ -551,7 +581,7 @@ hipError_t enqueue_step(const Plan &p, int flip, unsigned long long *counters, C
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         hipLaunchKernelGGL(kernel, dim3(vgrid), dim3(64), lds, ctx->stream, subs, p.vprefix,
-                           p.count, counters + kHdiffQueues);
+                           p.count, counters + kPipeCounter);
         return hipGetLastError();
     };
     if (pf == 4) return nt ? launch_v(vadv_batch_kernel<T, C, 4, true>) : launch_v(vadv_batch_kernel<T, C, 4, false>);
