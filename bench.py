@@ -67,19 +67,27 @@ def device_fields_random(domain, halo, dtype, count, seed, alignment=64):
     return fields, geometry(domain, strides, halo)
 
 
+# Per-launch times come from events the dispatch records itself (hipExtLaunch-
+# KernelGGL: the reference's 'hip-ext' timer).  Events recorded on the stream
+# around the launch add about 2 us of marker latency per launch (measured on
+# MI355X: hdiff 512x512x160 171.9 -> 169.5 us, vadv 256x256x80 float32
+# storage 35.8 -> 33.7 us), which counts against the short kernels most.
+TIMER = native.SB_TIMER_HIP_EXT
+
+
 def time_launches(launch, steps, warmup, flush=True):
     """Warm up with `warmup` launches configured like the timed ones (cache
     flush included, so the flush buffer exists and the clocks have ramped),
     then time `steps` launches; returns per-launch seconds and wall."""
     for _ in range(warmup):
         seconds = ctypes.c_double(0.0)
-        launch(native.launch_options(flush=flush), ctypes.byref(seconds))
+        launch(native.launch_options(flush=flush, timer=TIMER), ctypes.byref(seconds))
     native.call("sb_synchronize")
     times = []
     start = time.perf_counter()
     for _ in range(steps):
         seconds = ctypes.c_double(0.0)
-        launch(native.launch_options(flush=flush), ctypes.byref(seconds))
+        launch(native.launch_options(flush=flush, timer=TIMER), ctypes.byref(seconds))
         times.append(seconds.value)
     native.call("sb_synchronize")
     wall = time.perf_counter() - start
@@ -158,7 +166,8 @@ def bench_stream(repeats, shapes=None):
     words = {"copy": 2, "scale": 2, "add": 3, "triad": 3}
     best, shape_of, table = {}, {}, []
     for block, vector, unroll, nt, per_cu in shapes or STREAM_SWEEP:
-        opts = native.launch_options(variant=nt, block=(block, vector, per_cu), unroll=unroll)
+        opts = native.launch_options(variant=nt, block=(block, vector, per_cu), unroll=unroll,
+                                     timer=TIMER)
         for op, code in native.STREAM_OPS.items():
             times = []
             for _ in range(repeats + 1):
@@ -263,10 +272,10 @@ def bench_plain_steps(steps=10, storage=np.float64, compute=np.float64):
         refs = [ctypes.byref(x) if timed else None for x in seconds]
         native.call("sb_hdiff", prec, ctypes.c_void_p(cur.pointer()),
                     ctypes.c_void_p(f["coeff"].pointer()), ctypes.c_void_p(nxt.pointer()),
-                    sub.geom, native.launch_options(), refs[0])
+                    sub.geom, native.launch_options(timer=TIMER), refs[0])
         ptrs = [nxt, f["upos"], f["utens"], f["utensstage"], f["wcon"]]
         arr = (ctypes.c_void_p * 8)(*([p.pointer() for p in ptrs] + [0, 0, 0]))
-        native.call("sb_vadv", prec, 0, arr, sub.geom, native.launch_options(), refs[1])
+        native.call("sb_vadv", prec, 0, arr, sub.geom, native.launch_options(timer=TIMER), refs[1])
         sub.advance(1)
         return seconds[0].value + seconds[1].value
 
