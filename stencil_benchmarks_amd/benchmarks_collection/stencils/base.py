@@ -283,10 +283,10 @@ class LaplacianStencil(BasicStencil):
 
     def setup(self):
         super().setup()
-        if any(h < 1 for h, on in zip(self.halo, self.along_axes) if on):
-            raise ParameterError(
-                f"positive horizontal halo size required (given halo: {self.halo})"
-            )
+        short = [axis for axis, (h, on) in enumerate(zip(self.halo, self.along_axes)) if on and h < 1]
+        if short:
+            raise ParameterError(f"the Laplacian needs a halo of 1 along axes {short}, "
+                                 f"got halo {self.halo}")
 
     def expected(self, inp):
         inp = self._oracle_input(inp)
@@ -327,10 +327,9 @@ def horizontal_diffusion(inp, coeff):
 class HorizontalDiffusionStencil(Stencil):
     def setup(self):
         super().setup()
-        if any(h < 2 for h in self.halo[:2]):
-            raise ParameterError(
-                f"horizontal halo size must be at least 2 (given halo: {self.halo})"
-            )
+        if min(self.halo[:2]) < 2:
+            raise ParameterError(f"horizontal diffusion reads two points beyond the domain in i "
+                                 f"and j: halo {self.halo} is too narrow")
 
     @property
     def args(self):
@@ -429,7 +428,8 @@ class VerticalAdvectionStencil(Stencil):
             raise ParameterError("vertical advection needs at least 2 vertical levels")
         super().setup()
         if self.halo[0] < 1 or (self.all_components and any(h < 1 for h in self.halo)):
-            raise ParameterError(f"positive halo size required (given halo: {self.halo})")
+            raise ParameterError(f"vertical advection reads wcon one point beyond the domain: "
+                                 f"halo {self.halo} is too narrow")
 
     @property
     def components(self):

@@ -80,7 +80,8 @@ class Native(Benchmark):
             raise ParameterError("workgroups per CU must be in 0..64")
         per_block = self.block_size * vector * self.unroll_factor
         if self.array_size % per_block:
-            warnings.warn("adapting array size to match block and vector sizes")
+            warnings.warn(f"array size rounded up to a multiple of {per_block} elements "
+                          "(block x vector x unroll)")
             self.array_size = -(-self.array_size // per_block) * per_block
         try:
             native.init(self.device)
