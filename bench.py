@@ -359,6 +359,29 @@ def host_threads():
     return int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
 
 
+def host_description():
+    """CPU model, the logical CPUs this process may run on and the physical
+    cores among them (from /proc/cpuinfo), for the CPU legs' report."""
+    allowed = os.sched_getaffinity(0)
+    model, cores = "unknown", set()
+    try:
+        entry = {}
+        for line in Path("/proc/cpuinfo").read_text().splitlines() + [""]:
+            if not line.strip():
+                if entry.get("processor") is not None and int(entry["processor"]) in allowed:
+                    model = entry.get("model name", model)
+                    cores.add((entry.get("physical id"), entry.get("core id")))
+                entry = {}
+                continue
+            key, _, value = line.partition(":")
+            entry[key.strip()] = value.strip()
+    except (OSError, ValueError):
+        pass
+    return {"model": model, "logical_cpus": len(allowed), "physical_cores": len(cores) or None,
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+
+
 def _time_cpu(call, budget_s, max_runs=5000):
     call()  # warm-up (first touch, thread spin-up)
     times, spent = [], 0.0
@@ -396,6 +419,7 @@ def cpu_baseline(headline, budget_s=12.0):
                   f"{runs} runs ({spent:.1f} s), median run",
         "seconds_per_pass": best,
         "max_abs_diff_vs_gpu": diff,
+        "host": host_description(),
     }
 
 
