@@ -40,11 +40,12 @@ __host__ __device__ constexpr int64_t hybrid_lds_rows(int64_t nz, int nr, int pf
 template <typename T, typename C, int NR, int PF, int AUX>
 struct HybridColumn {
     static_assert(NR % PF == 0 && NR >= PF, "register levels: whole prefetch blocks");
-    // With float32 fields the forward sweep keeps pos of the register levels
-    // (NR more 32-bit registers, within the wave's budget), so the backward
-    // sweep reads pos from memory for the LDS levels only; FP64 fields would
-    // need 2 NR more and stay at re-reading it.
-    static constexpr bool kKeepPos = sizeof(T) == 4;
+    // The forward sweep keeps pos of the lowest NK levels in registers (all
+    // NR register levels with float32 fields; 16 with FP64 fields, which
+    // fill the wave's 512 registers without scratch, where 8, 12 and 20
+    // spilled), so the backward sweep reads pos from memory for the levels
+    // above only.
+    static constexpr int NK = sizeof(T) == 4 ? NR : 16;
 
     // forward state entering a level: stage(k-1), stage(k), wcon and its
     // neighbour at k, c'(k-1), d'(k-1)
@@ -112,12 +113,12 @@ struct HybridColumn {
         for (int p = 0; p < PF; ++p) load(p, ring[p]);
 
         C cc[NR], dd[NR];
-        T pk[kKeepPos ? NR : 1];  // pos of the register levels (kKeepPos)
+        T pk[NK];  // pos of the lowest NK levels
         State s{}, h{};
         // ---- level 0 ---------------------------------------------------------
         {
             const Level<T> r = ring[0];
-            if constexpr (kKeepPos) pk[0] = r.pos;
+            pk[0] = r.pos;
             load(PF, ring[0]);
             const C gcv = C(0.25) * (C(r.wn) + C(r.w));
             const C cs = gcv * bet_m;
@@ -134,7 +135,7 @@ struct HybridColumn {
         for (int k = 1; k < NR; ++k) {
             h = hold(k == kl, s, h);
             const Level<T> r = ring[k % PF];
-            if constexpr (kKeepPos) pk[k] = r.pos;
+            if (k < NK) pk[k] = r.pos;
             load(k + PF, ring[k % PF]);
             middle(r, s, cc[k], dd[k]);
         }
@@ -170,10 +171,10 @@ struct HybridColumn {
         // ---- backward: LDS levels from the top, then the register levels ------
         // slot p of the pos ring holds the level of in-block position p
         T pr[PF];
-        // a level below NR: kept in pk (kKeepPos), so its load is dropped
-        // (an out-of-range offset reads nothing and returns 0)
+        // a level below NK is kept in pk, so its load is dropped (an
+        // out-of-range offset reads nothing and returns 0)
         auto pos_load = [&](int k) {
-            const bool kept = kKeepPos && k < NR;
+            const bool kept = k < NK;
             return bload<T, 0>(R.pos, kept ? kRegDropOffset : F.lane, kept ? 0u : off(k));
         };
 #pragma unroll
@@ -196,11 +197,11 @@ struct HybridColumn {
         for (int k = NR - 1; k >= 0; --k) {
             const int p = k % PF;
             T pos;
-            if constexpr (kKeepPos) {
+            if (k < NK) {
                 pos = pk[k];
             } else {
                 pos = pr[p];
-                if (k >= PF) pr[p] = bload<T, 0>(R.pos, F.lane, off(k - PF));
+                pr[p] = pos_load(k - PF);
             }
             const bool live = k < kl;
             const C xn = dd[k] - cc[k] * x;
