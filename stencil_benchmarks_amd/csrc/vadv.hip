@@ -392,14 +392,17 @@ void vadv_hybrid_kernel(Components comps, const T *__restrict__ wcon, PipeArgs a
     extern __shared__ unsigned char lds_raw[];
     constexpr int AUX = NT ? 2 : 0;
     CD<C> *lds = reinterpret_cast<CD<C> *>(lds_raw) + (threadIdx.x & 63);
-    const int64_t g = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
-    if (g >= a.ngroups) return;
-    for (int c = 0; c < a.ncomp; ++c) {
-        const Component &cp = comps.c[c];
-        const Rsrc R{make_rsrc(cp.stage, a.span_bytes), make_rsrc(cp.pos, a.span_bytes),
-                     make_rsrc(cp.tens, a.span_bytes), make_rsrc(cp.tensstage, a.span_bytes),
-                     make_rsrc(wcon, a.span_bytes)};
-        HybridColumn<T, C, hybrid_nr<C>(), hybrid_pf<T>(), AUX>::solve(R, cp.wshift, a, g, lds);
+    // rounds of gridDim.x consecutive groups, consecutive groups of a round on
+    // one XCD; a grid of every resident wave walks the rounds
+    const int64_t slot = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
+    for (int64_t g = slot; g < a.ngroups; g += gridDim.x) {
+        for (int c = 0; c < a.ncomp; ++c) {
+            const Component &cp = comps.c[c];
+            const Rsrc R{make_rsrc(cp.stage, a.span_bytes), make_rsrc(cp.pos, a.span_bytes),
+                         make_rsrc(cp.tens, a.span_bytes), make_rsrc(cp.tensstage, a.span_bytes),
+                         make_rsrc(wcon, a.span_bytes)};
+            HybridColumn<T, C, hybrid_nr<C>(), hybrid_pf<T>(), AUX>::solve(R, cp.wshift, a, g, lds);
+        }
     }
 }
 
@@ -412,15 +415,18 @@ template <typename T, typename C>
 hipError_t launch_hybrid(bool nt, uint64_t groups, size_t lds_bytes, const Components &comps,
                          const T *wcon, const PipeArgs &pa, Context *ctx) {
     PipeArgs args = pa;
-    args.xcds = groups >= uint64_t(ctx->xcds) ? uint32_t(ctx->xcds) : 1u;
+    // one wave per SIMD (its registers), as many per CU as the LDS allows
+    const uint64_t per_cu = lds_bytes ? std::min<uint64_t>(4, ctx->lds_per_block / lds_bytes) : 4;
+    const uint64_t grid = std::min<uint64_t>(groups, std::max<uint64_t>(1, per_cu) * ctx->compute_units);
+    args.xcds = grid >= uint64_t(ctx->xcds) ? uint32_t(ctx->xcds) : 1u;
     const void *fn = nt ? reinterpret_cast<const void *>(&vadv_hybrid_kernel<T, C, true>)
                         : reinterpret_cast<const void *>(&vadv_hybrid_kernel<T, C, false>);
     (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes));
     if (nt)
-        enqueue_kernel((vadv_hybrid_kernel<T, C, true>), dim3(unsigned(groups)), dim3(64),
+        enqueue_kernel((vadv_hybrid_kernel<T, C, true>), dim3(unsigned(grid)), dim3(64),
                        lds_bytes, ctx->stream, comps, wcon, args);
     else
-        enqueue_kernel((vadv_hybrid_kernel<T, C, false>), dim3(unsigned(groups)), dim3(64),
+        enqueue_kernel((vadv_hybrid_kernel<T, C, false>), dim3(unsigned(grid)), dim3(64),
                        lds_bytes, ctx->stream, comps, wcon, args);
     return hipGetLastError();
 }

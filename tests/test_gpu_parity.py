@@ -279,6 +279,20 @@ def test_vadv_one_wave_register_columns(gpu, prec, domain, nt):
     assert np.array_equal(got[inner(domain, halo)], expected)
 
 
+# more column groups than the resident waves hold at once: each wave walks
+# several rounds (1280 groups at nz 40 against 1024 waves; 1680 groups at
+# nz 96, where two waves fit a CU)
+@pytest.mark.parametrize("prec", PRECISIONS)
+@pytest.mark.parametrize("domain", [(320, 256, 40), (448, 240, 96)])
+def test_vadv_one_wave_register_columns_several_rounds(gpu, prec, domain):
+    halo = (1, 0, 0)
+    dtype = _dtype(prec)
+    fields = make_fields(domain, halo, dtype, count=8, seed=sum(domain))
+    got, = _run_vadv(gpu, fields, [3], domain, halo, _prec(gpu, prec), False, 5)
+    expected = _expected(lambda *f: oracle.vadv(*f, domain, halo), fields[:5], prec)
+    assert np.array_equal(got[inner(domain, halo)], expected)
+
+
 def test_vadv_one_wave_rejects_columns_beyond_lds(gpu):
     from stencil_benchmarks_amd.benchmark import ExecutionError
 
