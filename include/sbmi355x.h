@@ -178,7 +178,7 @@ int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
  * (the Thomas coefficients stay on chip, or in a library-owned scratch for
  * variant 2).  Requires hi >= 1 (all components: hj >= 1 too).  Variants:
  * 0 default (5 where nz <= 96 and every 64-column group is in flight at
- * once; else 4 for nz >= 96, or nz >= 64 with float32 storage, with at
+ * once; else 4 for nz >= 96, or nz >= 48 with float32 storage, with at
  * least two column groups per resident workgroup, else 3; beyond the LDS
  * limit of those (nz > 160 with FP64 arithmetic) 5 up to nz = 201, then 2),
  * 1 one wave per

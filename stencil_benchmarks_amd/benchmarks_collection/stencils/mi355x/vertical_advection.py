@@ -5,7 +5,7 @@ backward substitution in one kernel (csrc/vadv.hip).  ``Fused`` picks the
 solver: one wave per 64-column group with the lowest levels' Thomas
 coefficients in registers where every group of the domain fits on the chip
 at once (nz <= 96), else the warp-specialised one (four waves per group)
-for nz >= 96 (nz >= 64 with float32 storage) and the single-wave pipeline
+for nz >= 96 (nz >= 48 with float32 storage) and the single-wave pipeline
 below that, both with the Thomas coefficients in LDS (nz <= 160 in FP64
 arithmetic), and a global FP64 scratch beyond.  ``LdsColumn``,
 ``GlobalScratch``, ``Pipelined``, ``WarpSpecialised`` and

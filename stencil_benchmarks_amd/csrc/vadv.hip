@@ -639,10 +639,12 @@ int launch_vadv(int all_components, void *const *f, const sb_geom *g, const sb_l
     // default: the warp-specialised solver where every workgroup gets at
     // least two groups and the columns are tall enough: nz >= 96 for FP64
     // storage (the single-wave pipeline is then held to one group per CU by
-    // LDS), nz >= 64 for float32 storage (measured on MI355X: 512x512x160
-    // FP64 1.18x, float32 storage 1.9x; 256x256x80 FP64 0.96x, float32
-    // storage 1.20x; 512x512x64 float32 storage 1.24x)
-    const int ws_min_nz = sizeof(T) == 8 ? 96 : 64;
+    // LDS), nz >= 48 for float32 storage (measured on MI355X against the
+    // pipeline: 512x512x160 FP64 1.18x, float32 storage 1.9x; 256x256x80
+    // FP64 0.96x, float32 storage 1.20x; float32 storage 512x512x64 1.24x,
+    // 512x512x56 1.21x, 320x320x56 1.19x, 384x384x48 1.09x, 512x512x48
+    // 1.07x, 512x512x40 0.84x)
+    const int ws_min_nz = sizeof(T) == 8 ? 96 : 48;
     const bool ws_default = ws_fits && buffer_ok &&
                             uint64_t(last_elem + 1 + g->sk) * sizeof(T) < kWsDropOffset &&
                             g->nz >= ws_min_nz && groups >= 2 * int64_t(ctx->compute_units);
