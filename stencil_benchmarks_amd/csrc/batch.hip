@@ -32,21 +32,22 @@
 //     buffer sized for the tallest column of its launch.  The host orders
 //     the items (subdomains by decreasing item cost from a model of the
 //     solver's pipeline, (groups + 1) x (levels / M + 3) barrier steps per
-//     item, residues in order) and cuts the list into one contiguous range
-//     of about equal cost per XCD; workgroups draw tickets from their XCD's
-//     range, then from the others', like the hdiff items.  Consecutive
+//     item, residues in order), deals them in a snake over the resident
+//     workgroup slots and gives each XCD's queue the items of its slots in
+//     dealing order (schedule_ws); workgroups draw tickets from their XCD's
+//     queue, then from the others', like the hdiff items.  Consecutive
 //     residues (adjacent groups of a row) then run on one XCD at about the
-//     same time, so the waves of the chip touch the same few k planes (and
-//     pages) at a time, as in the plain kernel; runs of consecutive groups
-//     per workgroup scatter the waves over many planes and ran 3x slower.
+//     same time, so the waves of the chip touch the same few k planes at a
+//     time, as in the plain kernel; runs of consecutive groups per
+//     workgroup scatter the waves over many planes and ran 3x slower.
 //     Subdomains whose slot buffer fits twice in a CU's LDS (nz <= 80 with
 //     FP64 arithmetic) run in a launch of their own at two workgroups per
 //     CU, the taller ones at one (measured on MI355X: 28 % faster for
 //     float32-storage columns of nz <= 80).
-// The counters are cleared by a small kernel at the start of every
-// sequence (no memset nodes, so the sequence can be captured in a
-// hipGraph and replayed).  Every workgroup exits once the counters pass
-// their item counts.
+// The counters (one 128-byte line each) are cleared by a small kernel at
+// the start of every sequence (no memset nodes, so the sequence can be
+// captured in a hipGraph and replayed).  Every workgroup exits once the
+// counters pass their item counts.
 #include <algorithm>
 #include <cstring>
 #include <memory>
