@@ -144,7 +144,9 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
     using Tile = HdiffTile<T, C, TY, RPT, FLAGS>;
     __shared__ C s_in[Tile::IN_ELEMS];
     __shared__ C s_lap[Tile::LAP_ELEMS];
-    __shared__ long long s_item;
+    // tickets, two slots used in turn: item n's ticket is read from one
+    // while item n publishes item n+1's into the other
+    __shared__ long long s_item[2];
     const int64_t total = prefix[count];
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const int home = int(blockIdx.x % uint32_t(queues));
@@ -152,21 +154,25 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
     // mostly stay in one subdomain, which then needs no search
     int sub = 0;
     int64_t sub_lo = 0, sub_hi = 0;
+    int slot = 0;
     for (int r = 0; r < queues; ++r) {
         const int q = home + r < queues ? home + r : home + r - queues;
         const int64_t lo = total * q / queues, hi = total * (q + 1) / queues;
-        // the next item's ticket is drawn while the current item runs, so the
-        // atomic's round trip overlaps the tile's loads; every ticket drawn
-        // is either worked or ends this range
-        unsigned long long next = 0;
-        if (tid == 0) next = atomicAdd(counters + q * kCounterStride, 1ull);
-        for (;;) {
-            if (tid == 0) s_item = lo + (long long)next;
-            __syncthreads();
-            const int64_t item = uniform64(uint64_t(s_item));
-            __syncthreads();
-            if (item >= hi) break;
-            if (tid == 0) next = atomicAdd(counters + q * kCounterStride, 1ull);
+        unsigned long long *const counter = counters + q * kCounterStride;
+        __syncthreads();  // every wave has read the last ticket of the previous range
+        if (tid == 0) s_item[slot] = lo + (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        int64_t item = uniform64(uint64_t(s_item[slot]));
+        while (item < hi) {
+            // The following item's ticket is drawn as this item starts and
+            // published from inside the tile worker once the first plane's
+            // loads, issued after the atomic, have landed: the atomic's round
+            // trip hides behind them, and no wave waits for the item's stores
+            // to read it (the result is used in every lane there, so no later
+            // instruction waits on its register).  Every ticket drawn is
+            // either worked or ends this range.
+            unsigned long long next = 0;
+            if (tid == 0) next = atomicAdd(counter, 1ull);
             if (item < sub_lo || item >= sub_hi) {
                 sub = find_sub(prefix, count, item);
                 sub_lo = prefix[sub];
@@ -182,12 +188,23 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
             const int64_t k_begin = kc * KC;
             const int64_t k_end = min(k_begin + int64_t(KC), d.nz);
             const HdiffGeom g{d.nx, d.ny, d.nz, d.si, d.sj, d.sk};
+            const int following = slot ^ 1;
             // descriptor pointers have no known address space: pass them as global
             // pointers so the tile's loads are global (flat loads would also count
             // against the LDS wait counter)
             Tile::run(as_global(static_cast<const T *>(d.inp)),
                       as_global(static_cast<const T *>(d.coeff)), as_global(static_cast<T *>(d.out)),
-                      g, i0, j0, k_begin, k_end, s_in, s_lap);
+                      g, i0, j0, k_begin, k_end, s_in, s_lap, [&]() {
+                          const long long ticket = lo + (long long)next;
+                          // an empty statement that reads the ticket in every
+                          // lane keeps the add (and the wait for the atomic)
+                          // out of the one-lane branch below
+                          asm volatile("" : : "v"(ticket));
+                          if (tid == 0) s_item[following] = ticket;
+                      });
+            __syncthreads();  // the tile's LDS is free and the ticket visible
+            slot = following;
+            item = uniform64(uint64_t(s_item[slot]));
         }
     }
 }

@@ -58,6 +58,11 @@ __device__ __forceinline__ T ld_nt(global_ptr<const T> p) {
         return *p;
 }
 
+// The tile worker's default hook: nothing.
+struct NoTileHook {
+    __device__ void operator()() const {}
+};
+
 template <bool NT, typename T>
 __device__ __forceinline__ void st_nt(global_ptr<T> p, T v) {
     if constexpr (NT)
@@ -77,6 +82,8 @@ __device__ __forceinline__ void st_nt(global_ptr<T> p, T v) {
 // computed and written to the other half of the double-buffered LDS tile
 // afterwards.  FLAGS bit 0: non-temporal out stores; bit 1: non-temporal
 // coeff loads (neither is re-read; inp is, by the neighbouring halos).
+// `hook` runs once per call, in every thread, right after the first plane
+// is staged (its loads have been waited on), before any barrier.
 template <typename T, typename C, int TY, int RPT, int FLAGS>
 struct HdiffTile {
     static constexpr int TX = 64;
@@ -88,10 +95,12 @@ struct HdiffTile {
     static constexpr int IN_ELEMS = 2 * LH * LW;  // double-buffered input tile
     static constexpr int LAP_ELEMS = TY * PW;
 
+    template <typename Hook = NoTileHook>
     __device__ static void run(global_ptr<const T> __restrict__ inp,
                                global_ptr<const T> __restrict__ coeff, global_ptr<T> __restrict__ out,
                                const HdiffGeom &a, int64_t i0, int64_t j0,
-                               int64_t k_begin, int64_t k_end, C *s_in_base, C *s_lap) {
+                               int64_t k_begin, int64_t k_end, C *s_in_base, C *s_lap,
+                               Hook hook = Hook{}) {
         C(*s_in)[LH * LW] = reinterpret_cast<C(*)[LH * LW]>(s_in_base);
         const int tx = threadIdx.x, ty = threadIdx.y;
         const int tid = ty * TX + tx;
@@ -150,9 +159,13 @@ struct HdiffTile {
             if (has_halo) s_in[buf][lds_halo] = C(ph);
         };
 
-        if (k_begin >= k_end) return;
+        if (k_begin >= k_end) {
+            hook();
+            return;
+        }
         issue(k_begin);
         commit(0);
+        hook();
         const int col = tx + 2;         // staged column of this thread
         const int row0 = ty * RPT + 2;  // staged row of its first output
         for (int64_t k = k_begin; k < k_end; ++k) {
