@@ -3,8 +3,9 @@
 The other GPU tests sweep small and ragged shapes; these run the shapes the
 measurements are quoted on: horizontal diffusion and the x/y Laplacian at
 512x512x160 (halo 3, 64-byte alignment, layout (2, 1, 0)), vertical
-advection at 512x512x160 and 256x256x80, STREAM at 2**29 elements and the
-96-subdomain batch, and the served request mix.  The stencils are compared
+advection at 512x512x160 and 256x256x80, STREAM at 2**29 elements, the
+96-subdomain batch, a batch of one 512x512x160 subdomain, and the served
+request mix.  The stencils are compared
 with the C oracle bit for bit (FP64, and float32 storage with FP64
 arithmetic against the FP64 oracle on the float32 inputs, rounded once);
 STREAM with the reference's closed-form recurrence; the batch and the
@@ -185,6 +186,30 @@ def test_batch_of_96_subdomains_equals_plain_kernels(gpu, storage):
     finally:
         for sub in batched:
             sub.free()
+
+
+@pytest.mark.parametrize("planes_per_item", [1, 2])
+def test_batch_of_one_512x512x160_subdomain_equals_plain_kernels(gpu, planes_per_item):
+    # one large subdomain: 40,960 or 20,480 hdiff items over the per-XCD
+    # queues, so every workgroup draws many tickets and steals from the
+    # other queues at the end
+    from stencil_benchmarks_amd import batch
+
+    domain, halo = (512, 512, 160), (3, 3, 3)
+    batched = batch.Subdomain(domain, halo, "float64", seed=7)
+    plain = batch.Subdomain(domain, halo, "float64", seed=7)
+    try:
+        options = gpu.launch_options(block=(0, 16, planes_per_item))
+        for _ in range(2):
+            batch.step([batched], compute="float64", hdiff_options=options)
+            _plain_step(gpu, plain, gpu.SB_F64)
+        gpu.call("sb_synchronize")
+        sl = inner(domain, halo)
+        assert np.array_equal(batched.state()[sl], plain.state()[sl])
+        assert np.array_equal(batched.download("utensstage")[sl], plain.download("utensstage")[sl])
+    finally:
+        batched.free()
+        plain.free()
 
 
 def test_served_request_mix_equals_plain_kernels(gpu):
