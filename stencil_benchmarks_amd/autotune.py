@@ -56,9 +56,10 @@ BATCH_VADV_SPACE = (
     {"variant": 4, "by": 16, "unroll": 8, "bx": 5},
     {"variant": 3, "unroll": 8}, {"variant": 3, "unroll": 4},
 )
-# plain hdiff: tile rows x k-planes x cache hints (->unroll 0: non-temporal
-# coeff loads; else 4 | bits: bit 0 non-temporal out stores, bit 1
-# non-temporal coeff loads)
+# plain hdiff: register-staged tile rows x k-planes x cache hints (->unroll
+# 0: non-temporal coeff loads; else 4 | bits: bit 0 non-temporal out stores,
+# bit 1 non-temporal coeff loads); tune_hdiff adds the LDS-DMA tile's two
+# heights (variant 2) for FP64 domains of whole tiles
 PLAIN_HDIFF_SPACE = tuple({"variant": 1, "by": by, "bz": bz, "unroll": u}
                           for by in (8, 16, 32) for bz in (1, 2, 4, 8, 16) for u in (0, 4, 5, 7))
 # plain vadv: variant 4 (->unroll levels per barrier step) or 3 (->unroll
@@ -248,7 +249,11 @@ def tune_hdiff(domain, storage, compute=None, halo=(3, 3, 3), cache=None, repeat
         for _ in range(warmup):
             native.call("sb_hdiff", code, *ptrs, geom, native.launch_options(), None)
 
-        h, _, seconds, trials = descend(PLAIN_HDIFF_SPACE, (), measure, {})
+        space = PLAIN_HDIFF_SPACE
+        if storage == np.float64 and compute == np.float64 and domain[0] % 64 == 0:
+            # the LDS-DMA tile (variant 2) takes whole 64 x 8 / 64 x 16 tiles
+            space = space + tuple({"variant": 2, "by": by} for by in (8, 16) if domain[1] % by == 0)
+        h, _, seconds, trials = descend(space, (), measure, {})
     finally:
         fields[0].free()
     cache.put(key, {"hdiff": h, "vadv": {}, "seconds": seconds, "trials": len(trials)})

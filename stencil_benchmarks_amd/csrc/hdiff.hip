@@ -18,11 +18,25 @@
 // HBM; each output point costs one inp read (plus the halo share), one
 // coeff read and one out write.
 //
+// Variant 2 (LDS-DMA tile; FP64, default wherever it applies).  The same
+// arithmetic (HdiffTile::plane) on a 64 x TY tile of one k-plane per
+// workgroup, its (64+4) x (TY+4) input tile staged straight into LDS by
+// 16-byte global-to-LDS loads: the tile is one contiguous LDS array whose
+// 16-byte chunks are pairs of adjacent doubles of a row, so wave instruction
+// n stages chunks 64n .. 64n+63 and no VGPR holds the staged data.  With
+// 64 x 8 tiles and 2 rows per thread a workgroup needs 58 VGPRs and 11.4 KB
+// of LDS, so eight share a CU and overlap one another's staging and
+// arithmetic.  It takes whole tiles only (nx a multiple of 64, ny of TY),
+// unit i stride, even row and plane strides and a 16-byte aligned interior
+// origin; other domains take variant 1.
+//
 // Variant 3 (on the fly).  One thread per point reading its 13 inputs
 // through the caches; the schedule of the reference's OnTheFly body
 // (stencils/cuda_hip/horizontal_diffusion.py:50-96) with the oracle's
 // arithmetic order.  Kept as an independent implementation to cross-check
 // variant 1.
+#include <type_traits>
+
 #include "hdiff_tile.h"
 
 namespace sb {
@@ -66,6 +80,73 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_lds_kernel(
     const HdiffGeom g{a.nx, a.ny, a.nz, a.si, a.sj, a.sk};
     Tile::run(as_global(inp), as_global(coeff), as_global(out), g, i0, j0, k_begin, k_end, s_in,
               s_lap);
+}
+
+// Variant 2 kernel (see the file comment).  Chunk c of the staged tile holds
+// columns 2 (c % (LW/2)) and +1 of row c / (LW/2), counted from point
+// (i0-2, j0-2); the coeff loads are non-temporal (not re-read), the stores
+// plain.
+template <int TY, int RPT>
+__global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_dma_kernel(
+    const double *__restrict__ inp, const double *__restrict__ coeff, double *__restrict__ out,
+    HdiffArgs a) {
+    using Tile = HdiffTile<double, double, TY, RPT, 2>;
+    constexpr int LW = Tile::LW, LH = Tile::LH, NW = Tile::NW;
+    constexpr int CHUNKS = LH * LW / 2;              // 16-byte chunks of the staged tile
+    constexpr int STEPS = (CHUNKS + 63) / 64;        // wave instructions that stage it
+    __shared__ __attribute__((aligned(16))) double s_tile[STEPS * 128];
+    __shared__ double s_lap[Tile::LAP_ELEMS];
+    const uint32_t tiles = uint32_t(a.tiles_i) * uint32_t(a.tiles_j);
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
+    const uint32_t k = b / tiles;
+    const uint32_t t = b - k * tiles;
+    const int64_t i0 = int64_t(t % uint32_t(a.tiles_i)) * Tile::TX;
+    const int64_t j0 = int64_t(t / uint32_t(a.tiles_i)) * TY;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int tid = ty * Tile::TX + tx;
+    using GB = __attribute__((address_space(1))) const char *;
+    using GP = __attribute__((address_space(1))) const void *;
+    using LP = __attribute__((address_space(3))) void *;
+    const GB corner = (GB)(inp) + ((int64_t(k) * a.sk + (j0 - 2) * a.sj + (i0 - 2)) * 8);
+#pragma unroll
+    for (int n = 0; n < STEPS; ++n) {
+        if (n % NW != ty) continue;  // wave-uniform: instruction n belongs to wave n % NW
+        const int c = n * 64 + tx;
+        if (c < CHUNKS) {
+            const int row = c / (LW / 2), pair = c - row * (LW / 2);
+            __builtin_amdgcn_global_load_lds((GP)(corner + (int64_t(row) * a.sj + 2 * pair) * 8),
+                                             (LP)(s_tile + n * 128), 16, 0, 0);
+        }
+    }
+    using GD = __attribute__((address_space(1))) const double *;
+    const int64_t point = int64_t(k) * a.sk + (j0 + ty * RPT) * a.sj + i0 + tx;
+    double cf[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) cf[q] = __builtin_nontemporal_load((GD)(coeff) + point + q * a.sj);
+    __syncthreads();  // every wave's staging has landed
+    double res[RPT];
+    Tile::plane(s_tile, s_lap, cf, res, tx, ty, tid);
+    using GO = __attribute__((address_space(1))) double *;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) *((GO)(out) + point + q * a.sj) = res[q];
+}
+
+// Whether the variant-2 kernel with TY-row tiles takes this domain.
+bool hdiff_dma_fits(const sb_geom *g, const void *inp, int ty) {
+    return g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0 && g->sj % 2 == 0 && g->sk % 2 == 0 &&
+           reinterpret_cast<uintptr_t>(inp) % 16 == 0;
+}
+
+template <int TY, int RPT>
+hipError_t launch_dma(const double *inp, const double *coeff, double *out, HdiffArgs a,
+                      Context *ctx) {
+    a.tiles_i = int32_t(a.nx / 64);
+    a.tiles_j = int32_t(a.ny / TY);
+    const uint64_t blocks = uint64_t(a.tiles_i) * uint64_t(a.tiles_j) * uint64_t(a.nz);
+    if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
+    enqueue_kernel((hdiff_dma_kernel<TY, RPT>), dim3(unsigned(blocks)), dim3(64, TY / RPT), 0,
+                   ctx->stream, inp, coeff, out, a);
+    return hipGetLastError();
 }
 
 template <typename T, typename C>
@@ -153,8 +234,36 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
     HdiffArgs a{g->nx, g->ny, g->nz, g->si, g->sj, g->sk, 0, 0, ctx->xcds, 0};
     // the LDS tile (variant 1, default) addresses a plane with 32-bit offsets
     const bool tile_fits = hdiff_tile_fits(g->nx, g->ny, g->si, g->sj, sizeof(T));
+    constexpr bool fp64 = std::is_same_v<T, double> && std::is_same_v<C, double>;
+    // tile options (rows, planes, cache hints) select the register-staged tile
+    const bool tile_options = l && (l->by || l->bz || l->unroll);
     int variant = l ? l->variant : 0;
-    if (variant == 0) variant = tile_fits ? 1 : 3;
+    if (variant == 0) {
+        // the LDS-DMA tile where it applies (measured on MI355X, 512 x 512 x
+        // 160 FP64: 156.5-157.3 us against 168.7 us for variant 1's default)
+        if (fp64 && !tile_options && hdiff_dma_fits(g, inp, 8))
+            variant = 2;
+        else
+            variant = tile_fits ? 1 : 3;
+    }
+    if (variant == 2) {
+        SB_REQUIRE(fp64, "the LDS-DMA hdiff tile (variant 2) takes FP64 fields and arithmetic only");
+        const int ty2 = (l && l->by) ? l->by : 8;
+        SB_REQUIRE(ty2 == 8 || ty2 == 16, "LDS-DMA hdiff tile height %d not in {8, 16}", ty2);
+        SB_REQUIRE(!l || (l->bz <= 1 && l->unroll == 0),
+                   "the LDS-DMA hdiff tile takes one k-plane per workgroup and no cache-hint options");
+        SB_REQUIRE(hdiff_dma_fits(g, inp, ty2),
+                   "the LDS-DMA hdiff tile needs nx a multiple of 64, ny a multiple of %d, unit i "
+                   "stride, even j and k strides and a 16-byte aligned interior origin",
+                   ty2);
+        return timed_launch(l, seconds, [&]() -> hipError_t {
+            if constexpr (fp64)
+                return ty2 == 8 ? launch_dma<8, 2>(inp, coeff, out, a, ctx)
+                                : launch_dma<16, 4>(inp, coeff, out, a, ctx);
+            else
+                return hipErrorInvalidValue;
+        });
+    }
     SB_REQUIRE(variant != 1 || tile_fits,
                "the LDS-tiled hdiff needs positive i and j strides and a k-plane below 4 GiB");
     const int ty = (l && l->by) ? l->by : 16;
