@@ -65,10 +65,12 @@ def basic_bandwidth(output, executions, option):
 @main.command()
 @_options
 def horizontal_diffusion_bandwidth(output, executions, option):
-    """The LDS-tiled kernel at each tile height and the on-the-fly kernel."""
+    """The LDS-tiled kernels at each tile height and the on-the-fly kernel."""
     kwargs = common_kwargs(option)
     configurations = [
         Configuration(hdiff.Fused, name="fused", **kwargs),
+        Configuration(hdiff.LdsDma, name="lds-dma-64x16", block_size=(64, 16, 0), **kwargs),
+        Configuration(hdiff.Fused, name="fused-64x16", block_size=(64, 16, 0), **kwargs),
         Configuration(hdiff.Fused, name="fused-64x8", block_size=(64, 8, 0), **kwargs),
         Configuration(hdiff.Fused, name="fused-64x32", block_size=(64, 32, 0), **kwargs),
         Configuration(hdiff.OnTheFly, name="on-the-fly", **kwargs),
