@@ -623,7 +623,7 @@ def main():
             "workload": "hdiff 512x512x160 halo 3 float64, 64-byte aligned, layout (2,1,0)",
             "domain": [512, 512, 160],
             "halo": [3, 3, 3],
-            "kernel": "sb_hdiff variant 1 (LDS tile 64x16)",
+            "kernel": "sb_hdiff default: variant 2 (64x8 LDS tile staged by global-to-LDS loads)",
             "compulsory_bytes_per_launch": head["bytes"],
             "device": info["name"] or f"{info['arch']} ({info['compute_units']} CUs)",
         },
