@@ -138,8 +138,12 @@ int sb_flush_caches(void);
  * 4/8/16), 2 marching kernel (a thread walks ->bz = 8/16/32/64 rows of the
  * second-smallest-stride axis keeping three rows in registers; needs
  * 16-byte vector access along the unit-stride axis and no term along the
- * largest-stride axis).  The default marches where a term lies along the
- * second axis. */
+ * largest-stride axis), 3 (sb_laplacian only: SB_F64 with terms along x and
+ * y only, si == 1, nx a multiple of 64 and ny of ->by) LDS-DMA tile: a
+ * 64 x ->by tile (8/16/32 rows, 0 = 16 where ny allows, else 8) of one
+ * k-plane per workgroup, staged by global-to-LDS loads.  The default is 3
+ * where it applies and no ->by/->bz is given, else marches where a term lies
+ * along the second axis. */
 int sb_copy(int prec, const void *inp, void *out, const sb_geom *g,
             const sb_launch *l, double *seconds);
 /* out = (inp[x + e_axis] + inp[x]) / 2 (base.py:191-205). */

@@ -272,6 +272,78 @@ __global__ __launch_bounds__(BLOCK) void march_kernel(const T *__restrict__ inp,
     }
 }
 
+// Laplacian along i and j in FP64 (variant 3, default where it applies): a
+// 64 x TY tile of one k-plane per workgroup, its (64+2) x (TY+2) input tile
+// staged straight into LDS by 4-byte global-to-LDS loads (one contiguous LDS
+// array; wave instruction n stages dwords 64n .. 64n+63 of it, so the tile's
+// 8-byte aligned corner at (i0-1, j0-1) needs no 16-byte alignment and no
+// second halo column), then each thread forms RPT points of one column:
+// r = 0; r += (2 c - in(i+1)) - in(i-1); r += (2 c - in(j+1)) - in(j-1).
+// One plane per workgroup and a few VGPRs: many workgroups per CU overlap one
+// another's staging, and the grid has many rounds, so no tail of idle CUs.
+struct LapDmaArgs {
+    int64_t sj, sk;
+    int32_t tiles_i, tiles_j, xcds, pad;
+};
+
+template <int TY, int RPT>
+__global__ __launch_bounds__(64 * (TY / RPT)) void lap_dma_kernel(const double *__restrict__ inp,
+                                                                  double *__restrict__ out,
+                                                                  LapDmaArgs a) {
+    constexpr int TX = 64, NW = TY / RPT, LW = TX + 2, LH = TY + 2;
+    constexpr int DWORDS = LH * LW * 2;          // staged tile in 4-byte words
+    constexpr int STEPS = (DWORDS + 63) / 64;    // wave instructions that stage it
+    __shared__ __attribute__((aligned(16))) double s_tile[STEPS * 32];
+    const uint32_t tiles = uint32_t(a.tiles_i) * uint32_t(a.tiles_j);
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
+    const uint32_t k = b / tiles;
+    const uint32_t t = b - k * tiles;
+    const int64_t i0 = int64_t(t % uint32_t(a.tiles_i)) * TX;
+    const int64_t j0 = int64_t(t / uint32_t(a.tiles_i)) * TY;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    using GB = __attribute__((address_space(1))) const char *;
+    using GP = __attribute__((address_space(1))) const void *;
+    using LP = __attribute__((address_space(3))) void *;
+    const GB corner = (GB)(inp) + ((int64_t(k) * a.sk + (j0 - 1) * a.sj + (i0 - 1)) * 8);
+#pragma unroll
+    for (int n = 0; n < STEPS; ++n) {
+        if (n % NW != ty) continue;  // wave-uniform: instruction n belongs to wave n % NW
+        const int d = n * 64 + tx;
+        if (d < DWORDS) {
+            const int row = d / (2 * LW), word = d - row * (2 * LW);
+            __builtin_amdgcn_global_load_lds((GP)(corner + int64_t(row) * a.sj * 8 + word * 4),
+                                             (LP)(s_tile + n * 32), 4, 0, 0);
+        }
+    }
+    __syncthreads();  // every wave's staging has landed
+    using GO = __attribute__((address_space(1))) double *;
+    const int64_t point = int64_t(k) * a.sk + (j0 + ty * RPT) * a.sj + i0 + tx;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int at = (ty * RPT + q + 1) * LW + tx + 1;
+        const double c = s_tile[at];
+        double r = 0.0;
+        r += (2.0 * c - s_tile[at + 1]) - s_tile[at - 1];
+        r += (2.0 * c - s_tile[at + LW]) - s_tile[at - LW];
+        *((GO)(out) + point + q * a.sj) = r;
+    }
+}
+
+// Whether the variant-3 Laplacian with TY-row tiles takes this call.
+bool lap_dma_fits(const sb_geom *g, int lap_mask, int ty) {
+    return lap_mask == 3 && g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0 && g->nz > 0;
+}
+
+template <int TY, int RPT>
+hipError_t launch_lap_dma(const double *inp, double *out, const sb_geom *g, Context *ctx) {
+    LapDmaArgs a{g->sj, g->sk, int32_t(g->nx / 64), int32_t(g->ny / TY), ctx->xcds, 0};
+    const uint64_t blocks = uint64_t(a.tiles_i) * uint64_t(a.tiles_j) * uint64_t(g->nz);
+    if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
+    enqueue_kernel((lap_dma_kernel<TY, RPT>), dim3(unsigned(blocks)), dim3(64, TY / RPT), 0,
+                   ctx->stream, inp, out, a);
+    return hipGetLastError();
+}
+
 template <int P, int OP>
 int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis, int lap_mask,
                       const sb_launch *l, double *seconds) {
@@ -335,7 +407,32 @@ int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis
         axis1 = a.axis == 1;
     }
     const int variant = l ? l->variant : 0;
-    SB_REQUIRE(variant >= 0 && variant <= 2, "basic stencil variant %d not in {0, 1, 2}", variant);
+    SB_REQUIRE(variant >= 0 && variant <= 3, "basic stencil variant %d not in {0, 1, 2, 3}", variant);
+    // launch->variant 3: the LDS-DMA Laplacian (FP64, terms along i and j,
+    // unit i stride, whole 64 x by tiles; launch->by = tile rows 8, 16 or 32,
+    // 0 = 16 where ny allows, else 8), the default for such calls without
+    // row options.  512x512x160: 107 us with 16 rows, 110 us with 32, 125 us
+    // with 8, against 131 us for the marching kernel.
+    constexpr bool lap64 = OP == OP_LAP && std::is_same_v<T, double> && std::is_same_v<C, double>;
+    SB_REQUIRE(variant != 3 || lap64,
+               "basic stencil variant 3 (LDS-DMA tile) takes FP64 Laplacians only");
+    if constexpr (lap64) {
+        const int ty = (l && l->by) ? l->by : (g->ny % 16 == 0 ? 16 : 8);
+        const bool row_options = l && (l->by || l->bz);
+        if (variant == 3 || (variant == 0 && !row_options && lap_dma_fits(g, lap_mask, ty))) {
+            SB_REQUIRE(ty == 8 || ty == 16 || ty == 32, "LDS-DMA Laplacian tile rows %d not in {8, 16, 32}",
+                       ty);
+            SB_REQUIRE(lap_dma_fits(g, lap_mask, ty),
+                       "the LDS-DMA Laplacian takes terms along i and j only, unit i stride, nx a "
+                       "multiple of 64 and ny a multiple of %d",
+                       ty);
+            return timed_launch(l, seconds, [&]() -> hipError_t {
+                if (ty == 8) return launch_lap_dma<8, 2>(inp, out, g, ctx);
+                if (ty == 16) return launch_lap_dma<16, 4>(inp, out, g, ctx);
+                return launch_lap_dma<32, 8>(inp, out, g, ctx);
+            });
+        }
+    }
     SB_REQUIRE(variant != 2 || marchable,
                "the marching kernel needs vector access along the unit-stride axis and no "
                "term along the largest-stride axis");

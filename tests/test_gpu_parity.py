@@ -159,6 +159,52 @@ def test_marching_kernel_rejects_axis_2_terms(gpu):
         _run(gpu, "sb_laplacian", [inp, out], [1], domain, halo, gpu.SB_F64, 1, 1, 1, launch=opts)
 
 
+@pytest.mark.parametrize("tile", [0, 8, 16, 32])
+@pytest.mark.parametrize("domain", [(64, 32, 1), (128, 96, 3), (192, 32, 2), (64, 64, 5)])
+@pytest.mark.parametrize("halo,alignment", [((1, 1, 0), 0), ((3, 3, 3), 64), ((1, 2, 1), 8)])
+def test_laplacian_lds_dma_tile(gpu, tile, domain, halo, alignment):
+    # variant 3: FP64 x/y Laplacian on 64 x tile tiles staged by global-to-LDS
+    # loads; the tile's corner is 8-byte aligned only (halo 1, no padding)
+    inp, = make_fields(domain, halo, np.float64, (2, 1, 0), alignment, seed=sum(domain) + tile)
+    out = _with_sentinel(make_fields(domain, halo, np.float64, (2, 1, 0), alignment)[0])
+    opts = gpu.launch_options(variant=3, block=(0, tile, 0))
+    got, = _run(gpu, "sb_laplacian", [inp, out], [1], domain, halo, gpu.SB_F64, 1, 1, 0,
+                launch=opts)
+    assert np.array_equal(got[inner(domain, halo)], oracle.laplacian(inp, (1, 1, 0), domain, halo))
+    _check_halo(got, domain, halo)
+
+
+@pytest.mark.parametrize("domain", [(64, 16, 3), (64, 24, 3), (128, 8, 2)])
+def test_laplacian_default_tile_rows(gpu, domain):
+    # the default takes 16-row tiles where ny allows, else 8-row ones
+    halo = (1, 1, 1)
+    inp, = make_fields(domain, halo, np.float64, (2, 1, 0), 0, seed=sum(domain))
+    out = _with_sentinel(make_fields(domain, halo, np.float64, (2, 1, 0), 0)[0])
+    got, = _run(gpu, "sb_laplacian", [inp, out], [1], domain, halo, gpu.SB_F64, 1, 1, 0)
+    assert np.array_equal(got[inner(domain, halo)], oracle.laplacian(inp, (1, 1, 0), domain, halo))
+    _check_halo(got, domain, halo)
+
+
+@pytest.mark.parametrize("domain,prec,mask,tile,match", [
+    ((70, 16, 2), "f64", (1, 1, 0), 0, "multiple of 64"),
+    ((64, 12, 2), "f64", (1, 1, 0), 0, "multiple of 8"),
+    ((64, 16, 2), "f64", (1, 1, 0), 32, "multiple of 32"),
+    ((64, 16, 2), "f64", (1, 1, 0), 4, "tile rows 4"),
+    ((64, 16, 2), "f64", (1, 1, 1), 0, "along i and j"),
+    ((64, 16, 2), "f64", (1, 0, 0), 0, "along i and j"),
+    ((64, 16, 2), "f32_f64", (1, 1, 0), 0, "FP64")])
+def test_laplacian_lds_dma_tile_rejects(gpu, domain, prec, mask, tile, match):
+    from stencil_benchmarks_amd.benchmark import ExecutionError
+
+    halo = (1, 1, 1)
+    dtype = _dtype(prec)
+    inp, out = make_fields(domain, halo, dtype, count=2)
+    opts = gpu.launch_options(variant=3, block=(0, tile, 0))
+    with pytest.raises(ExecutionError, match=match):
+        _run(gpu, "sb_laplacian", [inp, out], [1], domain, halo, _prec(gpu, prec), *mask,
+             launch=opts)
+
+
 HDIFF_DOMAINS = [(70, 37, 9), (64, 16, 1), (1, 1, 1), (129, 33, 3), (5, 70, 2)]
 
 
