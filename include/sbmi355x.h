@@ -162,16 +162,18 @@ int sb_laplacian(int prec, int along_x, int along_y, int along_z,
  * (stencils/cuda_hip/horizontal_diffusion.py:50-128 and
  * templates/horizontal_diffusion_*.j2); semantics of base.py:276-311.
  * Requires hi, hj >= 2.  Variants: 0 default (2 for SB_F64 domains of
- * whole 64 x 8 tiles when no tile option is given, else 1 where positive i
+ * whole 64 x 8 tiles and SB_F32 domains of whole 64 x 16 tiles when no
+ * tile option is given, else 1 where positive i
  * and j strides and a k-plane below 4 GiB let the tile use 32-bit plane
  * offsets, else 3), 1 LDS-staged tile
  * (launch->by = tile rows 8/16/32, ->bz = k-planes per workgroup 1..16, 0
  * measured default; ->unroll 0 = default (non-temporal coeff loads), else
  * 4 | flags with bit 0 non-temporal out stores, bit 1 non-temporal coeff
  * loads), 2 LDS tile staged by global-to-LDS loads, one k-plane per
- * workgroup (SB_F64 only; launch->by = tile rows 8 or 16, 0 = 8; needs nx a
- * multiple of 64, ny of the tile rows, si = 1, even sj and sk and a
- * 16-byte aligned inp), 3 per-point recomputation through the caches
+ * workgroup (every precision; launch->by = tile rows 8 or 16, 0 = 8 for
+ * SB_F64 and 16 for float32 fields; needs nx a multiple of 64, ny of the
+ * tile rows, si = 1, and for SB_F64 even sj and sk and a 16-byte aligned
+ * inp), 3 per-point recomputation through the caches
  * (the reference's OnTheFly schedule). */
 int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,
              const sb_geom *g, const sb_launch *l, double *seconds);

@@ -250,8 +250,9 @@ def tune_hdiff(domain, storage, compute=None, halo=(3, 3, 3), cache=None, repeat
             native.call("sb_hdiff", code, *ptrs, geom, native.launch_options(), None)
 
         space = PLAIN_HDIFF_SPACE
-        if storage == np.float64 and compute == np.float64 and domain[0] % 64 == 0:
+        if domain[0] % 64 == 0:
             # the LDS-DMA tile (variant 2) takes whole 64 x 8 / 64 x 16 tiles
+            # (the tuning fields have 64-byte aligned rows, as FP64 needs)
             space = space + tuple({"variant": 2, "by": by} for by in (8, 16) if domain[1] % by == 0)
         h, _, seconds, trials = descend(space, (), measure, {})
     finally:

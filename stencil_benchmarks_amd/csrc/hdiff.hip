@@ -18,17 +18,19 @@
 // HBM; each output point costs one inp read (plus the halo share), one
 // coeff read and one out write.
 //
-// Variant 2 (LDS-DMA tile; FP64, default wherever it applies).  The same
-// arithmetic (HdiffTile::plane) on a 64 x TY tile of one k-plane per
-// workgroup, its (64+4) x (TY+4) input tile staged straight into LDS by
-// 16-byte global-to-LDS loads: the tile is one contiguous LDS array whose
-// 16-byte chunks are pairs of adjacent doubles of a row, so wave instruction
-// n stages chunks 64n .. 64n+63 and no VGPR holds the staged data.  With
-// 64 x 8 tiles and 2 rows per thread a workgroup needs 58 VGPRs and 11.4 KB
-// of LDS, so eight share a CU and overlap one another's staging and
-// arithmetic.  It takes whole tiles only (nx a multiple of 64, ny of TY),
-// unit i stride, even row and plane strides and a 16-byte aligned interior
-// origin; other domains take variant 1.
+// Variant 2 (LDS-DMA tile; default wherever it applies for FP64 and for
+// FP32 fields and arithmetic).  The same arithmetic (HdiffTile::plane) on a
+// 64 x TY tile of one k-plane per workgroup, its (64+4) x (TY+4) input tile
+// staged straight into LDS by global-to-LDS loads: the tile is one
+// contiguous LDS array, so wave instruction n stages units 64n .. 64n+63
+// of it and no VGPR holds the staged data.  A unit is a 16-byte chunk (two
+// adjacent doubles of a row) for FP64 fields and one 4-byte element for
+// float32 fields.  With 64 x 8 FP64 tiles and 2 rows per thread a workgroup
+// needs 54 VGPRs and 11.4 KB of LDS, so eight share a CU and overlap one
+// another's staging and arithmetic.  It takes whole tiles only (nx a
+// multiple of 64, ny of TY) and unit i stride, and for FP64 fields even row
+// and plane strides and a 16-byte aligned interior origin; other domains
+// take variant 1.
 //
 // Variant 3 (on the fly).  One thread per point reading its 13 inputs
 // through the caches; the schedule of the reference's OnTheFly body
@@ -82,20 +84,25 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_lds_kernel(
               s_lap);
 }
 
-// Variant 2 kernel (see the file comment).  Chunk c of the staged tile holds
-// columns 2 (c % (LW/2)) and +1 of row c / (LW/2), counted from point
-// (i0-2, j0-2); the coeff loads are non-temporal (not re-read), the stores
-// plain.
-template <int TY, int RPT>
+// Variant 2 kernel (see the file comment).  The staged tile is one
+// contiguous LDS array counted from point (i0-2, j0-2), LW elements a row;
+// lane l of wave instruction n stages unit 64n + l of it.  A unit is a
+// 16-byte chunk (two adjacent doubles of a row) for 8-byte fields, whose
+// tile corner is then 16-byte aligned, and a 4-byte word (one element) for
+// float32 fields, which need no alignment beyond the element's.  The coeff
+// loads are non-temporal (not re-read), the stores plain.
+template <typename T, typename C, int TY, int RPT>
 __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_dma_kernel(
-    const double *__restrict__ inp, const double *__restrict__ coeff, double *__restrict__ out,
-    HdiffArgs a) {
-    using Tile = HdiffTile<double, double, TY, RPT, 2>;
+    const T *__restrict__ inp, const T *__restrict__ coeff, T *__restrict__ out, HdiffArgs a) {
+    using Tile = HdiffTile<T, C, TY, RPT, 2>;
     constexpr int LW = Tile::LW, LH = Tile::LH, NW = Tile::NW;
-    constexpr int CHUNKS = LH * LW / 2;              // 16-byte chunks of the staged tile
-    constexpr int STEPS = (CHUNKS + 63) / 64;        // wave instructions that stage it
-    __shared__ __attribute__((aligned(16))) double s_tile[STEPS * 128];
-    __shared__ double s_lap[Tile::LAP_ELEMS];
+    constexpr int UNIT = sizeof(T) == 8 ? 16 : 4;   // bytes a lane stages per instruction
+    constexpr int PER = UNIT / int(sizeof(T));      // elements per unit: 2 or 1
+    constexpr int UNITS = LH * LW / PER;            // units of the staged tile
+    constexpr int STEPS = (UNITS + 63) / 64;        // wave instructions that stage it
+    static_assert(LW % PER == 0, "a unit must not straddle two rows");
+    __shared__ __attribute__((aligned(16))) T s_tile[STEPS * 64 * PER];
+    __shared__ C s_lap[Tile::LAP_ELEMS];
     const uint32_t tiles = uint32_t(a.tiles_i) * uint32_t(a.tiles_j);
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
     const uint32_t k = b / tiles;
@@ -107,44 +114,50 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_dma_kernel(
     using GB = __attribute__((address_space(1))) const char *;
     using GP = __attribute__((address_space(1))) const void *;
     using LP = __attribute__((address_space(3))) void *;
-    const GB corner = (GB)(inp) + ((int64_t(k) * a.sk + (j0 - 2) * a.sj + (i0 - 2)) * 8);
+    const GB corner =
+        (GB)(inp) + ((int64_t(k) * a.sk + (j0 - 2) * a.sj + (i0 - 2)) * int64_t(sizeof(T)));
 #pragma unroll
     for (int n = 0; n < STEPS; ++n) {
         if (n % NW != ty) continue;  // wave-uniform: instruction n belongs to wave n % NW
         const int c = n * 64 + tx;
-        if (c < CHUNKS) {
-            const int row = c / (LW / 2), pair = c - row * (LW / 2);
-            __builtin_amdgcn_global_load_lds((GP)(corner + (int64_t(row) * a.sj + 2 * pair) * 8),
-                                             (LP)(s_tile + n * 128), 16, 0, 0);
+        if (c < UNITS) {
+            const int row = c / (LW / PER), unit = c - row * (LW / PER);
+            const GP src = (GP)(corner + (int64_t(row) * a.sj + PER * unit) * int64_t(sizeof(T)));
+            if constexpr (UNIT == 16)
+                __builtin_amdgcn_global_load_lds(src, (LP)(s_tile + n * 64 * PER), 16, 0, 0);
+            else
+                __builtin_amdgcn_global_load_lds(src, (LP)(s_tile + n * 64 * PER), 4, 0, 0);
         }
     }
-    using GD = __attribute__((address_space(1))) const double *;
+    using GT = __attribute__((address_space(1))) const T *;
     const int64_t point = int64_t(k) * a.sk + (j0 + ty * RPT) * a.sj + i0 + tx;
-    double cf[RPT];
+    C cf[RPT];
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) cf[q] = __builtin_nontemporal_load((GD)(coeff) + point + q * a.sj);
+    for (int q = 0; q < RPT; ++q) cf[q] = C(__builtin_nontemporal_load((GT)(coeff) + point + q * a.sj));
     __syncthreads();  // every wave's staging has landed
-    double res[RPT];
+    C res[RPT];
     Tile::plane(s_tile, s_lap, cf, res, tx, ty, tid);
-    using GO = __attribute__((address_space(1))) double *;
+    using GO = __attribute__((address_space(1))) T *;
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) *((GO)(out) + point + q * a.sj) = res[q];
+    for (int q = 0; q < RPT; ++q) *((GO)(out) + point + q * a.sj) = T(res[q]);
 }
 
-// Whether the variant-2 kernel with TY-row tiles takes this domain.
-bool hdiff_dma_fits(const sb_geom *g, const void *inp, int ty) {
-    return g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0 && g->sj % 2 == 0 && g->sk % 2 == 0 &&
-           reinterpret_cast<uintptr_t>(inp) % 16 == 0;
+// Whether the variant-2 kernel with TY-row tiles takes this domain: whole
+// tiles, unit i stride, and for 8-byte fields (16-byte units) even row and
+// plane strides and a 16-byte aligned interior origin.
+bool hdiff_dma_fits(const sb_geom *g, const void *inp, int ty, size_t itemsize) {
+    const bool tiles = g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0;
+    if (itemsize != 8) return tiles;
+    return tiles && g->sj % 2 == 0 && g->sk % 2 == 0 && reinterpret_cast<uintptr_t>(inp) % 16 == 0;
 }
 
-template <int TY, int RPT>
-hipError_t launch_dma(const double *inp, const double *coeff, double *out, HdiffArgs a,
-                      Context *ctx) {
+template <typename T, typename C, int TY, int RPT>
+hipError_t launch_dma(const T *inp, const T *coeff, T *out, HdiffArgs a, Context *ctx) {
     a.tiles_i = int32_t(a.nx / 64);
     a.tiles_j = int32_t(a.ny / TY);
     const uint64_t blocks = uint64_t(a.tiles_i) * uint64_t(a.tiles_j) * uint64_t(a.nz);
     if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
-    enqueue_kernel((hdiff_dma_kernel<TY, RPT>), dim3(unsigned(blocks)), dim3(64, TY / RPT), 0,
+    enqueue_kernel((hdiff_dma_kernel<T, C, TY, RPT>), dim3(unsigned(blocks)), dim3(64, TY / RPT), 0,
                    ctx->stream, inp, coeff, out, a);
     return hipGetLastError();
 }
@@ -235,33 +248,38 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
     // the LDS tile (variant 1, default) addresses a plane with 32-bit offsets
     const bool tile_fits = hdiff_tile_fits(g->nx, g->ny, g->si, g->sj, sizeof(T));
     constexpr bool fp64 = std::is_same_v<T, double> && std::is_same_v<C, double>;
+    constexpr bool fp32 = std::is_same_v<T, float> && std::is_same_v<C, float>;
+    // LDS-DMA tile rows when none are given: 8 for 8-byte fields, 16 for
+    // float32 ones (512 x 512 x 160, FP32: 82.0 us with 16 rows, 118.5 us
+    // with 8; float32 storage with FP64 arithmetic 100.0 / 132.4 us)
+    constexpr int dma_rows = sizeof(T) == 8 ? 8 : 16;
     // tile options (rows, planes, cache hints) select the register-staged tile
     const bool tile_options = l && (l->by || l->bz || l->unroll);
     int variant = l ? l->variant : 0;
     if (variant == 0) {
-        // the LDS-DMA tile where it applies (measured on MI355X, 512 x 512 x
-        // 160 FP64: 156.5-157.3 us against 168.7 us for variant 1's default)
-        if (fp64 && !tile_options && hdiff_dma_fits(g, inp, 8))
+        // the LDS-DMA tile where it applies, in FP64 and in FP32 (measured on
+        // MI355X at 512 x 512 x 160: FP64 156.5-157.3 us against 168.7 us for
+        // variant 1's default, FP32 82.0 us against 93.0-93.4 us); float32
+        // storage with FP64 arithmetic keeps variant 1 (98.0-99.3 us), whose
+        // 8 planes per workgroup spend fewer FP64 operations on halo rows
+        if ((fp64 || fp32) && !tile_options && hdiff_dma_fits(g, inp, dma_rows, sizeof(T)))
             variant = 2;
         else
             variant = tile_fits ? 1 : 3;
     }
     if (variant == 2) {
-        SB_REQUIRE(fp64, "the LDS-DMA hdiff tile (variant 2) takes FP64 fields and arithmetic only");
-        const int ty2 = (l && l->by) ? l->by : 8;
+        const int ty2 = (l && l->by) ? l->by : dma_rows;
         SB_REQUIRE(ty2 == 8 || ty2 == 16, "LDS-DMA hdiff tile height %d not in {8, 16}", ty2);
         SB_REQUIRE(!l || (l->bz <= 1 && l->unroll == 0),
                    "the LDS-DMA hdiff tile takes one k-plane per workgroup and no cache-hint options");
-        SB_REQUIRE(hdiff_dma_fits(g, inp, ty2),
+        SB_REQUIRE(hdiff_dma_fits(g, inp, ty2, sizeof(T)),
                    "the LDS-DMA hdiff tile needs nx a multiple of 64, ny a multiple of %d, unit i "
-                   "stride, even j and k strides and a 16-byte aligned interior origin",
+                   "stride, and for FP64 fields even j and k strides and a 16-byte aligned "
+                   "interior origin",
                    ty2);
         return timed_launch(l, seconds, [&]() -> hipError_t {
-            if constexpr (fp64)
-                return ty2 == 8 ? launch_dma<8, 2>(inp, coeff, out, a, ctx)
-                                : launch_dma<16, 4>(inp, coeff, out, a, ctx);
-            else
-                return hipErrorInvalidValue;
+            return ty2 == 8 ? launch_dma<T, C, 8, 2>(inp, coeff, out, a, ctx)
+                            : launch_dma<T, C, 16, 4>(inp, coeff, out, a, ctx);
         });
     }
     SB_REQUIRE(variant != 1 || tile_fits,

@@ -97,7 +97,9 @@ struct HdiffTile {
 
     // Laplacian, limited fluxes and update of the calling thread's RPT output
     // points (column tx, rows ty*RPT ..) from one staged plane `sin` (LW
-    // doubles per row, rows -2 .. TY+1, columns -2 .. TX+1).  Each thread
+    // elements of type S per row, rows -2 .. TY+1, columns -2 .. TX+1; each
+    // read widened to the arithmetic type C, so a plane staged in the storage
+    // type gives the same results as one staged in C).  Each thread
     // computes its column's Laplacian for rows -1..RPT in registers and
     // publishes the interior ones to s_lap for its i-neighbours; the
     // workgroup barrier inside separates those writes from the flux reads.
@@ -106,17 +108,18 @@ struct HdiffTile {
     // which keeps its own copy: calling this from there changes the register
     // allocation of the float32-storage kernel (126 -> 130 VGPRs, one wave
     // per SIMD fewer, 100 -> 110 us at 512x512x160).
-    __device__ static void plane(const C *sin, C *s_lap, const C (&cf)[RPT], C (&res)[RPT], int tx,
+    template <typename S>
+    __device__ static void plane(const S *sin, C *s_lap, const C (&cf)[RPT], C (&res)[RPT], int tx,
                                  int ty, int tid) {
         const int col = tx + 2;         // staged column of this thread
         const int row0 = ty * RPT + 2;  // staged row of its first output
         C ic[RPT + 4], il[RPT + 2], ir[RPT + 2], lap[RPT + 2];
     #pragma unroll
-        for (int q = 0; q < RPT + 4; ++q) ic[q] = sin[(row0 - 2 + q) * LW + col];
+        for (int q = 0; q < RPT + 4; ++q) ic[q] = C(sin[(row0 - 2 + q) * LW + col]);
     #pragma unroll
         for (int q = 0; q < RPT + 2; ++q) {
-            il[q] = sin[(row0 - 1 + q) * LW + col - 1];
-            ir[q] = sin[(row0 - 1 + q) * LW + col + 1];
+            il[q] = C(sin[(row0 - 1 + q) * LW + col - 1]);
+            ir[q] = C(sin[(row0 - 1 + q) * LW + col + 1]);
         }
     #pragma unroll
         for (int q = 0; q < RPT + 2; ++q)
@@ -128,9 +131,10 @@ struct HdiffTile {
             const int side = tid / TY, r = tid - side * TY;
             const int c = side ? TX + 2 : 1;
             const int rr = r + 2;
-            const C v = C(4) * sin[rr * LW + c] -
-                        (((sin[rr * LW + c + 1] + sin[rr * LW + c - 1]) + sin[(rr + 1) * LW + c]) +
-                         sin[(rr - 1) * LW + c]);
+            const C v = C(4) * C(sin[rr * LW + c]) -
+                        (((C(sin[rr * LW + c + 1]) + C(sin[rr * LW + c - 1])) +
+                          C(sin[(rr + 1) * LW + c])) +
+                         C(sin[(rr - 1) * LW + c]));
             s_lap[r * PW + (side ? TX + 1 : 0)] = v;
         }
         __syncthreads();
