@@ -3,9 +3,9 @@
 ``Fused`` is the LDS-tiled fused kernel (csrc/hdiff.hip): the input tile
 and its 2-point halo are staged in LDS and the Laplacian, flux limiter and
 update never round-trip HBM.  With the default block size the library picks
-the tile: for FP64 domains of whole 64 x 8 tiles and FP32 domains of whole
-64 x 16 tiles the one staged by global-to-LDS loads (variant 2, also
-``LdsDma``, which takes every precision), else the register-staged tile
+the tile: for FP64 domains of whole 64 x 8 tiles and float32 domains of
+whole 64 x 16 tiles the one staged by global-to-LDS loads (variant 2, also
+``LdsDma``), else the register-staged tile
 (variant 1), which an explicit tile height or plane count selects.
 ``OnTheFly`` recomputes every neighbour Laplacian from cached loads, like the
 reference's OnTheFly schedule (stencils/cuda_hip/horizontal_diffusion.py:

@@ -289,29 +289,32 @@ struct LapDmaArgs {
 template <int TY, int RPT>
 __global__ __launch_bounds__(64 * (TY / RPT)) void lap_dma_kernel(const double *__restrict__ inp,
                                                                   double *__restrict__ out,
-                                                                  LapDmaArgs a) {
+                                                                  LapDmaArgs a, TileGrid tg) {
     constexpr int TX = 64, NW = TY / RPT, LW = TX + 2, LH = TY + 2;
     constexpr int DWORDS = LH * LW * 2;          // staged tile in 4-byte words
     constexpr int STEPS = (DWORDS + 63) / 64;    // wave instructions that stage it
     __shared__ __attribute__((aligned(16))) double s_tile[STEPS * 32];
-    const uint32_t tiles = uint32_t(a.tiles_i) * uint32_t(a.tiles_j);
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
-    const uint32_t k = b / tiles;
-    const uint32_t t = b - k * tiles;
-    const int64_t i0 = int64_t(t % uint32_t(a.tiles_i)) * TX;
-    const int64_t j0 = int64_t(t / uint32_t(a.tiles_i)) * TY;
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    uint32_t ti, tj, k;
+    tile_of(blockIdx.x, tg, ti, tj, k);
+    const int64_t i0 = int64_t(ti) * TX;
+    const int64_t j0 = int64_t(tj) * TY;
+    // the wave index read as wave-uniform: the branches on it and the row
+    // offsets derived from it are scalar
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     using GB = __attribute__((address_space(1))) const char *;
     using GP = __attribute__((address_space(1))) const void *;
     using LP = __attribute__((address_space(3))) void *;
     const GB corner = (GB)(inp) + ((int64_t(k) * a.sk + (j0 - 1) * a.sj + (i0 - 1)) * 8);
+    // row pitch below 2^24 bytes (checked on the host)
+    const uint32_t pitch = uint32_t(a.sj) * 8u;
 #pragma unroll
     for (int n = 0; n < STEPS; ++n) {
-        if (n % NW != ty) continue;  // wave-uniform: instruction n belongs to wave n % NW
+        if (n % NW != ty) continue;  // instruction n belongs to wave n % NW
         const int d = n * 64 + tx;
         if (d < DWORDS) {
-            const int row = d / (2 * LW), word = d - row * (2 * LW);
-            __builtin_amdgcn_global_load_lds((GP)(corner + int64_t(row) * a.sj * 8 + word * 4),
+            const uint32_t row = uint32_t(d) / (2 * LW);
+            const uint32_t word = uint32_t(d) - __umul24(row, uint32_t(2 * LW));
+            __builtin_amdgcn_global_load_lds((GP)(corner + (__umul24(row, pitch) + word * 4u)),
                                              (LP)(s_tile + n * 32), 4, 0, 0);
         }
     }
@@ -331,7 +334,8 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void lap_dma_kernel(const double *
 
 // Whether the variant-3 Laplacian with TY-row tiles takes this call.
 bool lap_dma_fits(const sb_geom *g, int lap_mask, int ty) {
-    return lap_mask == 3 && g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0 && g->nz > 0;
+    return lap_mask == 3 && g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0 && g->nz > 0 &&
+           g->sj * 8 < (int64_t(1) << 24);
 }
 
 template <int TY, int RPT>
@@ -339,8 +343,10 @@ hipError_t launch_lap_dma(const double *inp, double *out, const sb_geom *g, Cont
     LapDmaArgs a{g->sj, g->sk, int32_t(g->nx / 64), int32_t(g->ny / TY), ctx->xcds, 0};
     const uint64_t blocks = uint64_t(a.tiles_i) * uint64_t(a.tiles_j) * uint64_t(g->nz);
     if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
+    const TileGrid tg = make_tile_grid(uint32_t(blocks), uint32_t(a.tiles_i), uint32_t(a.tiles_j),
+                                       ctx->xcds);
     enqueue_kernel((lap_dma_kernel<TY, RPT>), dim3(unsigned(blocks)), dim3(64, TY / RPT), 0,
-                   ctx->stream, inp, out, a);
+                   ctx->stream, inp, out, a, tg);
     return hipGetLastError();
 }
 
@@ -423,8 +429,8 @@ int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis
             SB_REQUIRE(ty == 8 || ty == 16 || ty == 32, "LDS-DMA Laplacian tile rows %d not in {8, 16, 32}",
                        ty);
             SB_REQUIRE(lap_dma_fits(g, lap_mask, ty),
-                       "the LDS-DMA Laplacian takes terms along i and j only, unit i stride, nx a "
-                       "multiple of 64 and ny a multiple of %d",
+                       "the LDS-DMA Laplacian takes terms along i and j only, unit i stride, a row "
+                       "pitch below 16 MiB, nx a multiple of 64 and ny a multiple of %d",
                        ty);
             return timed_launch(l, seconds, [&]() -> hipError_t {
                 if (ty == 8) return launch_lap_dma<8, 2>(inp, out, g, ctx);

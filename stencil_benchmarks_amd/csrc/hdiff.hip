@@ -18,8 +18,7 @@
 // HBM; each output point costs one inp read (plus the halo share), one
 // coeff read and one out write.
 //
-// Variant 2 (LDS-DMA tile; default wherever it applies for FP64 and for
-// FP32 fields and arithmetic).  The same arithmetic (HdiffTile::plane) on a
+// Variant 2 (LDS-DMA tile; default wherever it applies).  The same arithmetic (HdiffTile::plane) on a
 // 64 x TY tile of one k-plane per workgroup, its (64+4) x (TY+4) input tile
 // staged straight into LDS by global-to-LDS loads: the tile is one
 // contiguous LDS array, so wave instruction n stages units 64n .. 64n+63
@@ -93,7 +92,8 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_lds_kernel(
 // loads are non-temporal (not re-read), the stores plain.
 template <typename T, typename C, int TY, int RPT>
 __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_dma_kernel(
-    const T *__restrict__ inp, const T *__restrict__ coeff, T *__restrict__ out, HdiffArgs a) {
+    const T *__restrict__ inp, const T *__restrict__ coeff, T *__restrict__ out, HdiffArgs a,
+    TileGrid tg) {
     using Tile = HdiffTile<T, C, TY, RPT, 2>;
     constexpr int LW = Tile::LW, LH = Tile::LH, NW = Tile::NW;
     constexpr int UNIT = sizeof(T) == 8 ? 16 : 4;   // bytes a lane stages per instruction
@@ -103,26 +103,30 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_dma_kernel(
     static_assert(LW % PER == 0, "a unit must not straddle two rows");
     __shared__ __attribute__((aligned(16))) T s_tile[STEPS * 64 * PER];
     __shared__ C s_lap[Tile::LAP_ELEMS];
-    const uint32_t tiles = uint32_t(a.tiles_i) * uint32_t(a.tiles_j);
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x, a.xcds);
-    const uint32_t k = b / tiles;
-    const uint32_t t = b - k * tiles;
-    const int64_t i0 = int64_t(t % uint32_t(a.tiles_i)) * Tile::TX;
-    const int64_t j0 = int64_t(t / uint32_t(a.tiles_i)) * TY;
-    const int tx = threadIdx.x, ty = threadIdx.y;
+    uint32_t ti, tj, k;
+    tile_of(blockIdx.x, tg, ti, tj, k);
+    const int64_t i0 = int64_t(ti) * Tile::TX;
+    const int64_t j0 = int64_t(tj) * TY;
+    // the wave index read as wave-uniform: the branches on it and the row
+    // offsets derived from it are scalar
+    const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const int tid = ty * Tile::TX + tx;
     using GB = __attribute__((address_space(1))) const char *;
     using GP = __attribute__((address_space(1))) const void *;
     using LP = __attribute__((address_space(3))) void *;
     const GB corner =
         (GB)(inp) + ((int64_t(k) * a.sk + (j0 - 2) * a.sj + (i0 - 2)) * int64_t(sizeof(T)));
+    // row pitch below 2^24 bytes (checked on the host): a staged unit's byte
+    // offset from the tile corner is one 24-bit multiply and an add
+    const uint32_t pitch = uint32_t(a.sj) * uint32_t(sizeof(T));
 #pragma unroll
     for (int n = 0; n < STEPS; ++n) {
-        if (n % NW != ty) continue;  // wave-uniform: instruction n belongs to wave n % NW
+        if (n % NW != ty) continue;  // instruction n belongs to wave n % NW
         const int c = n * 64 + tx;
         if (c < UNITS) {
-            const int row = c / (LW / PER), unit = c - row * (LW / PER);
-            const GP src = (GP)(corner + (int64_t(row) * a.sj + PER * unit) * int64_t(sizeof(T)));
+            const uint32_t row = uint32_t(c) / (LW / PER);
+            const uint32_t unit = uint32_t(c) - __umul24(row, uint32_t(LW / PER));
+            const GP src = (GP)(corner + (__umul24(row, pitch) + unit * uint32_t(UNIT)));
             if constexpr (UNIT == 16)
                 __builtin_amdgcn_global_load_lds(src, (LP)(s_tile + n * 64 * PER), 16, 0, 0);
             else
@@ -143,10 +147,12 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_dma_kernel(
 }
 
 // Whether the variant-2 kernel with TY-row tiles takes this domain: whole
-// tiles, unit i stride, and for 8-byte fields (16-byte units) even row and
-// plane strides and a 16-byte aligned interior origin.
+// tiles, unit i stride, a row pitch below 16 MiB, and for 8-byte fields
+// (16-byte units) even row and plane strides and a 16-byte aligned interior
+// origin.
 bool hdiff_dma_fits(const sb_geom *g, const void *inp, int ty, size_t itemsize) {
-    const bool tiles = g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0;
+    const bool tiles = g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0 &&
+                       g->sj * int64_t(itemsize) < (int64_t(1) << 24);
     if (itemsize != 8) return tiles;
     return tiles && g->sj % 2 == 0 && g->sk % 2 == 0 && reinterpret_cast<uintptr_t>(inp) % 16 == 0;
 }
@@ -157,8 +163,10 @@ hipError_t launch_dma(const T *inp, const T *coeff, T *out, HdiffArgs a, Context
     a.tiles_j = int32_t(a.ny / TY);
     const uint64_t blocks = uint64_t(a.tiles_i) * uint64_t(a.tiles_j) * uint64_t(a.nz);
     if (blocks >= (uint64_t(1) << 31)) return hipErrorInvalidValue;
+    const TileGrid tg = make_tile_grid(uint32_t(blocks), uint32_t(a.tiles_i), uint32_t(a.tiles_j),
+                                       ctx->xcds);
     enqueue_kernel((hdiff_dma_kernel<T, C, TY, RPT>), dim3(unsigned(blocks)), dim3(64, TY / RPT), 0,
-                   ctx->stream, inp, coeff, out, a);
+                   ctx->stream, inp, coeff, out, a, tg);
     return hipGetLastError();
 }
 
@@ -247,22 +255,19 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
     HdiffArgs a{g->nx, g->ny, g->nz, g->si, g->sj, g->sk, 0, 0, ctx->xcds, 0};
     // the LDS tile (variant 1, default) addresses a plane with 32-bit offsets
     const bool tile_fits = hdiff_tile_fits(g->nx, g->ny, g->si, g->sj, sizeof(T));
-    constexpr bool fp64 = std::is_same_v<T, double> && std::is_same_v<C, double>;
-    constexpr bool fp32 = std::is_same_v<T, float> && std::is_same_v<C, float>;
     // LDS-DMA tile rows when none are given: 8 for 8-byte fields, 16 for
-    // float32 ones (512 x 512 x 160, FP32: 82.0 us with 16 rows, 118.5 us
-    // with 8; float32 storage with FP64 arithmetic 100.0 / 132.4 us)
+    // float32 ones (512 x 512 x 160, FP32: 80.7 us with 16 rows, 105 us
+    // with 8; float32 storage with FP64 arithmetic 86.0 / 116.4 us)
     constexpr int dma_rows = sizeof(T) == 8 ? 8 : 16;
     // tile options (rows, planes, cache hints) select the register-staged tile
     const bool tile_options = l && (l->by || l->bz || l->unroll);
     int variant = l ? l->variant : 0;
     if (variant == 0) {
-        // the LDS-DMA tile where it applies, in FP64 and in FP32 (measured on
-        // MI355X at 512 x 512 x 160: FP64 156.5-157.3 us against 168.7 us for
-        // variant 1's default, FP32 82.0 us against 93.0-93.4 us); float32
-        // storage with FP64 arithmetic keeps variant 1 (98.0-99.3 us), whose
-        // 8 planes per workgroup spend fewer FP64 operations on halo rows
-        if ((fp64 || fp32) && !tile_options && hdiff_dma_fits(g, inp, dma_rows, sizeof(T)))
+        // the LDS-DMA tile where it applies (measured on MI355X at 512 x 512
+        // x 160 against variant 1's defaults: FP64 156.5-157.3 us against
+        // 168.7 us, FP32 80.7 us against 93.0-93.4 us, float32 storage with
+        // FP64 arithmetic 86.0 us against 97.5-99.2 us)
+        if (!tile_options && hdiff_dma_fits(g, inp, dma_rows, sizeof(T)))
             variant = 2;
         else
             variant = tile_fits ? 1 : 3;
@@ -274,8 +279,8 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
                    "the LDS-DMA hdiff tile takes one k-plane per workgroup and no cache-hint options");
         SB_REQUIRE(hdiff_dma_fits(g, inp, ty2, sizeof(T)),
                    "the LDS-DMA hdiff tile needs nx a multiple of 64, ny a multiple of %d, unit i "
-                   "stride, and for FP64 fields even j and k strides and a 16-byte aligned "
-                   "interior origin",
+                   "stride, a row pitch below 16 MiB, and for FP64 fields even j and k strides "
+                   "and a 16-byte aligned interior origin",
                    ty2);
         return timed_launch(l, seconds, [&]() -> hipError_t {
             return ty2 == 8 ? launch_dma<T, C, 8, 2>(inp, coeff, out, a, ctx)

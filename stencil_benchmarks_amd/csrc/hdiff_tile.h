@@ -158,7 +158,9 @@ struct HdiffTile {
                                int64_t k_begin, int64_t k_end, C *s_in_base, C *s_lap,
                                Hook hook = Hook{}) {
         C(*s_in)[LH * LW] = reinterpret_cast<C(*)[LH * LW]>(s_in_base);
-        const int tx = threadIdx.x, ty = threadIdx.y;
+        // the wave index read as wave-uniform (every caller launches rows of
+        // 64 threads): branches on it and row offsets derived from it are scalar
+        const int tx = threadIdx.x, ty = __builtin_amdgcn_readfirstlane(threadIdx.y);
         const int tid = ty * TX + tx;
         // ---- per-thread staging offsets (plane-independent) -----------------
         // Byte offsets within a k-plane, 32-bit (a plane of a field below

@@ -149,6 +149,63 @@ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t total, uint32_t xcds) 
     return (b % xcds) * per + b / xcds;
 }
 
+// Division of a dividend below 2^31 by a divisor fixed at launch: q =
+// (n * mul) >> shift with mul = ceil(2^shift / d), shift = 31 + ceil(log2 d).
+// The rounding error of mul stays below 1/d of the quotient's unit for every
+// such n, so q = floor(n / d) exactly.  Wave-uniform dividends then take a
+// few scalar instructions instead of the VALU reciprocal sequence of a
+// division by a run-time divisor.
+struct UDiv {
+    uint64_t mul;
+    uint32_t shift, d;
+};
+
+inline UDiv make_udiv(uint32_t d) {
+    uint32_t l = 0;
+    while ((uint64_t(1) << l) < d) ++l;
+    const uint32_t shift = 31 + l;
+    return UDiv{((uint64_t(1) << shift) + d - 1) / d, shift, d};
+}
+
+__device__ inline uint32_t udiv(uint32_t n, const UDiv &u) {
+    return uint32_t((uint64_t(n) * u.mul) >> u.shift);
+}
+
+// The grid of a kernel with one workgroup per (tile_i, tile_j, plane), its
+// XCD remap (the one of xcd_remap, for a power-of-two XCD count) and tile
+// decomposition precomputed on the host, so a workgroup finds its tile with
+// scalar instructions.
+struct TileGrid {
+    uint32_t full;              // workgroups below it are remapped (0: none)
+    uint32_t per;               // full rounds: blocks / xcds
+    uint32_t xcd_mask, xcd_shift;
+    UDiv tiles, tiles_i;        // by tiles_i * tiles_j and by tiles_i
+};
+
+inline TileGrid make_tile_grid(uint32_t blocks, uint32_t tiles_i, uint32_t tiles_j, int xcds) {
+    TileGrid g{};
+    const bool pow2 = xcds > 1 && (xcds & (xcds - 1)) == 0;
+    if (pow2 && blocks >= uint32_t(xcds)) {
+        g.per = blocks / uint32_t(xcds);
+        g.full = g.per * uint32_t(xcds);
+        g.xcd_mask = uint32_t(xcds) - 1;
+        while ((1u << g.xcd_shift) < uint32_t(xcds)) ++g.xcd_shift;
+    }
+    g.tiles = make_udiv(tiles_i * tiles_j);
+    g.tiles_i = make_udiv(tiles_i);
+    return g;
+}
+
+// Tile (ti, tj) and plane k of workgroup b of such a grid.
+__device__ inline void tile_of(uint32_t b, const TileGrid &g, uint32_t &ti, uint32_t &tj,
+                               uint32_t &k) {
+    if (b < g.full) b = (b & g.xcd_mask) * g.per + (b >> g.xcd_shift);
+    k = udiv(b, g.tiles);
+    const uint32_t t = b - k * g.tiles.d;
+    tj = udiv(t, g.tiles_i);
+    ti = t - tj * g.tiles_i.d;
+}
+
 template <int P>
 struct Prec;
 template <>
