@@ -102,6 +102,8 @@ struct SubDev {
     int32_t tiles_i, tiles_j;
     uint32_t span_bytes;
     int32_t vstride;  // vadv items of this subdomain = group stride
+    // division by tiles_i * tiles_j and by tiles_i (hdiff item decode)
+    UDiv hdiv_tiles, hdiv_tiles_i;
 };
 
 // A wave-uniform copy of lane 0's value.  Work-queue tickets reach the other
@@ -179,13 +181,15 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_batch_kernel(
                 sub_hi = prefix[sub + 1];
             }
             const SubDev d = subs[sub];
-            const int64_t local = item - sub_lo;
-            const int64_t tiles = int64_t(d.tiles_i) * d.tiles_j;
-            const int64_t kc = local / tiles;
-            const int64_t t = local - kc * tiles;
-            const int64_t i0 = (t % d.tiles_i) * Tile::TX;
-            const int64_t j0 = (t / d.tiles_i) * TY;
-            const int64_t k_begin = kc * KC;
+            // a subdomain's items number below 2^30 (fields below 4 GiB),
+            // decoded by the host's multiply-shift constants in scalars
+            const uint32_t local = uint32_t(item - sub_lo);
+            const uint32_t kc = udiv(local, d.hdiv_tiles);
+            const uint32_t t = local - kc * d.hdiv_tiles.d;
+            const uint32_t tj = udiv(t, d.hdiv_tiles_i);
+            const int64_t i0 = int64_t(t - tj * d.hdiv_tiles_i.d) * Tile::TX;
+            const int64_t j0 = int64_t(tj) * TY;
+            const int64_t k_begin = int64_t(kc) * KC;
             const int64_t k_end = min(k_begin + int64_t(KC), d.nz);
             const HdiffGeom g{d.nx, d.ny, d.nz, d.si, d.sj, d.sk};
             const int following = slot ^ 1;
@@ -520,6 +524,8 @@ void build_tables(int prec, const sb_subdomain *subs, int count, const sb_launch
             d.sk = g.sk;
             d.tiles_i = int32_t(tiles_i);
             d.tiles_j = int32_t((g.ny + ty - 1) / ty);
+            d.hdiv_tiles = make_udiv(uint32_t(d.tiles_i) * uint32_t(d.tiles_j));
+            d.hdiv_tiles_i = make_udiv(uint32_t(d.tiles_i));
             d.span_bytes = uint32_t(uint64_t(last + 1) * esize);
             d.vstride = int32_t(vitems);
             p.host_subs[flip][r] = d;
