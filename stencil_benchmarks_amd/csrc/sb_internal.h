@@ -141,7 +141,7 @@ int vector_width(size_t elem_bytes, const void *const *ptrs, int nptrs,
 
 // Map a linear workgroup index so that consecutive "logical" tiles land on
 // the same XCD: hardware dispatches workgroup b to XCD b % xcds.
-__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t total, uint32_t xcds) {
+__host__ __device__ inline uint32_t xcd_remap(uint32_t b, uint32_t total, uint32_t xcds) {
     if (xcds <= 1 || total < xcds) return b;
     const uint32_t per = total / xcds;      // full rounds
     const uint32_t full = per * xcds;
@@ -167,7 +167,7 @@ inline UDiv make_udiv(uint32_t d) {
     return UDiv{((uint64_t(1) << shift) + d - 1) / d, shift, d};
 }
 
-__device__ inline uint32_t udiv(uint32_t n, const UDiv &u) {
+__host__ __device__ inline uint32_t udiv(uint32_t n, const UDiv &u) {
     return uint32_t((uint64_t(n) * u.mul) >> u.shift);
 }
 
@@ -197,7 +197,7 @@ inline TileGrid make_tile_grid(uint32_t blocks, uint32_t tiles_i, uint32_t tiles
 }
 
 // Tile (ti, tj) and plane k of workgroup b of such a grid.
-__device__ inline void tile_of(uint32_t b, const TileGrid &g, uint32_t &ti, uint32_t &tj,
+__host__ __device__ inline void tile_of(uint32_t b, const TileGrid &g, uint32_t &ti, uint32_t &tj,
                                uint32_t &k) {
     if (b < g.full) b = (b & g.xcd_mask) * g.per + (b >> g.xcd_shift);
     k = udiv(b, g.tiles);
