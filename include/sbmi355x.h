@@ -139,7 +139,7 @@ int sb_flush_caches(void);
  * second-smallest-stride axis keeping three rows in registers; needs
  * 16-byte vector access along the unit-stride axis and no term along the
  * largest-stride axis), 3 (sb_laplacian only: SB_F64 with terms along x and
- * y only, si == 1, a row pitch below 16 MiB, nx a multiple of 64 and ny of
+ * y only, si == 1, 0 < sj with a row pitch below 16 MiB, nx a multiple of 64 and ny of
  * ->by) LDS-DMA tile: a
  * 64 x ->by tile (8/16/32 rows, 0 = 16 where ny allows, else 8) of one
  * k-plane per workgroup, staged by global-to-LDS loads.  The default is 3
@@ -173,7 +173,7 @@ int sb_laplacian(int prec, int along_x, int along_y, int along_z,
  * loads), 2 LDS tile staged by global-to-LDS loads, one k-plane per
  * workgroup (every precision; launch->by = tile rows 8 or 16, 0 = 8 for
  * SB_F64 and 16 for float32 fields; needs nx a multiple of 64, ny of the
- * tile rows, si = 1, a row pitch below 16 MiB, and for SB_F64 even sj and
+ * tile rows, si = 1, 0 < sj with a row pitch below 16 MiB, and for SB_F64 even sj and
  * sk and a 16-byte aligned inp), 3 per-point recomputation through the caches
  * (the reference's OnTheFly schedule). */
 int sb_hdiff(int prec, const void *inp, const void *coeff, void *out,

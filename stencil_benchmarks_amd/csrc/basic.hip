@@ -335,7 +335,7 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void lap_dma_kernel(const double *
 // Whether the variant-3 Laplacian with TY-row tiles takes this call.
 bool lap_dma_fits(const sb_geom *g, int lap_mask, int ty) {
     return lap_mask == 3 && g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0 && g->nz > 0 &&
-           g->sj * 8 < (int64_t(1) << 24);
+           g->sj > 0 && g->sj * 8 < (int64_t(1) << 24);
 }
 
 template <int TY, int RPT>
@@ -429,8 +429,8 @@ int launch_basic_prec(const void *inp_v, void *out_v, const sb_geom *g, int axis
             SB_REQUIRE(ty == 8 || ty == 16 || ty == 32, "LDS-DMA Laplacian tile rows %d not in {8, 16, 32}",
                        ty);
             SB_REQUIRE(lap_dma_fits(g, lap_mask, ty),
-                       "the LDS-DMA Laplacian takes terms along i and j only, unit i stride, a row "
-                       "pitch below 16 MiB, nx a multiple of 64 and ny a multiple of %d",
+                       "the LDS-DMA Laplacian takes terms along i and j only, unit i stride, a "
+                       "positive row pitch below 16 MiB, nx a multiple of 64 and ny a multiple of %d",
                        ty);
             return timed_launch(l, seconds, [&]() -> hipError_t {
                 if (ty == 8) return launch_lap_dma<8, 2>(inp, out, g, ctx);

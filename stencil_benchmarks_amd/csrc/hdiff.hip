@@ -147,11 +147,11 @@ __global__ __launch_bounds__(64 * (TY / RPT)) void hdiff_dma_kernel(
 }
 
 // Whether the variant-2 kernel with TY-row tiles takes this domain: whole
-// tiles, unit i stride, a row pitch below 16 MiB, and for 8-byte fields
+// tiles, unit i stride, a positive row pitch below 16 MiB, and for 8-byte fields
 // (16-byte units) even row and plane strides and a 16-byte aligned interior
 // origin.
 bool hdiff_dma_fits(const sb_geom *g, const void *inp, int ty, size_t itemsize) {
-    const bool tiles = g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0 &&
+    const bool tiles = g->si == 1 && g->nx % 64 == 0 && g->ny % ty == 0 && g->sj > 0 &&
                        g->sj * int64_t(itemsize) < (int64_t(1) << 24);
     if (itemsize != 8) return tiles;
     return tiles && g->sj % 2 == 0 && g->sk % 2 == 0 && reinterpret_cast<uintptr_t>(inp) % 16 == 0;
@@ -279,7 +279,8 @@ int launch_hdiff(const void *inp_v, const void *coeff_v, void *out_v, const sb_g
                    "the LDS-DMA hdiff tile takes one k-plane per workgroup and no cache-hint options");
         SB_REQUIRE(hdiff_dma_fits(g, inp, ty2, sizeof(T)),
                    "the LDS-DMA hdiff tile needs nx a multiple of 64, ny a multiple of %d, unit i "
-                   "stride, a row pitch below 16 MiB, and for FP64 fields even j and k strides "
+                   "stride, a positive row pitch below 16 MiB, and for FP64 fields even j and k "
+                   "strides "
                    "and a 16-byte aligned interior origin",
                    ty2);
         return timed_launch(l, seconds, [&]() -> hipError_t {

@@ -205,6 +205,35 @@ def test_laplacian_lds_dma_tile_rejects(gpu, domain, prec, mask, tile, match):
              launch=opts)
 
 
+@pytest.mark.parametrize("entry", ["sb_hdiff", "sb_laplacian"])
+def test_lds_dma_tiles_reject_negative_row_stride(gpu, entry):
+    # the tiles address rows by 32-bit offsets from the tile corner: a flipped
+    # j axis (negative row stride) is refused, so the default, which asks the
+    # same check, takes another kernel for it
+    from stencil_benchmarks_amd.benchmark import ExecutionError
+    from stencil_benchmarks_amd.device import DeviceField, geometry
+
+    domain, halo = (64, 16, 2), (2, 2, 1)
+    fields = make_fields(domain, halo, np.float64, (2, 1, 0), 64, count=3)
+    devs = [DeviceField(f, halo) for f in fields]
+    try:
+        strides = [s // 8 for s in fields[0].strides]
+        geom = geometry(domain, [strides[0], -strides[1], strides[2]], halo)
+        # logical row 0 is the last interior row of the allocation
+        ptrs = [ctypes.c_void_p(d.pointer() + (domain[1] - 1) * strides[1] * 8) for d in devs]
+        if entry == "sb_hdiff":
+            args = (gpu.SB_F64, *ptrs)
+            opts = gpu.launch_options(variant=2)
+        else:
+            args = (gpu.SB_F64, 1, 1, 0, ptrs[0], ptrs[1])
+            opts = gpu.launch_options(variant=3)
+        with pytest.raises(ExecutionError, match="positive row pitch"):
+            gpu.call(entry, *args, geom, opts, None)
+    finally:
+        for d in devs:
+            d.free()
+
+
 HDIFF_DOMAINS = [(70, 37, 9), (64, 16, 1), (1, 1, 1), (129, 33, 3), (5, 70, 2)]
 
 
