@@ -2,8 +2,11 @@
 
 Copy, one-sided and symmetric averages and the Laplacian, computed only on
 the interior with the verification oracle's operation order (bit-exact in
-FP64).  The kernel vectorises along the unit-stride axis (dwordx4 loads
-when the pitches allow), see csrc/basic.hip.
+FP64).  The kernels vectorise along the unit-stride axis (dwordx4 loads
+when the pitches allow) or march along the second axis; the FP64 Laplacian
+along x and y on whole 64-column tiles takes a tile staged into LDS by
+global-to-LDS loads (``variant`` 3, the library's default there), see
+csrc/basic.hip.
 """
 
 import numpy as np
